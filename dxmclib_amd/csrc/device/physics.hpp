@@ -1,0 +1,799 @@
+// Device-side photon transport physics, templated on the arithmetic type T.
+//
+// Restates the reference hot path (include/dxmc/transport.hpp) for the GPU:
+//   sampling a photon from an exposure      exposure.hpp:280-304
+//   spectrum alias sampling                 dxmcrandom.hpp:211-216, 322-326
+//   bowtie / heel weights                   beamfilters.hpp:150-190, 514-538
+//   ray to world entry                      transport.hpp:702-728
+//   attenuation LUT and majorant            attenuationinterpolator.hpp:207-248
+//   photoelectric absorption                transport.hpp:216-263
+//   Rayleigh scatter (Thomson / RITA)       transport.hpp:264-298, dxmcrandom.hpp:483-500
+//   Compton scatter (KN, scatter factor)    transport.hpp:300-340, interpolation.hpp:169-176
+//   Compton impulse approximation           transport.hpp:342-483
+//   interaction dispatch, forced dispatch   transport.hpp:523-638
+//   Woodcock delta tracking                 transport.hpp:640-700
+//   direction deflection                    vectormath.hpp:83-99, 197-221
+// For T = double (the plain method) every expression keeps the reference's
+// operation order so the CPU oracle and this code round identically except
+// where the math library differs (log, pow, sin, cos, acos, exp); the file
+// including this header is compiled without FMA contraction. For T = float
+// the attenuation tables are evaluated in base 2 (exp2/log2 with host
+// re-based coefficients) instead of pow(10, .)/log10.
+//
+// Bounds: indices that the reference could push past an array end (voxel
+// index from a rounded position, the unclamped majorant index, a RITA draw
+// equal to the last CDF node) are clamped to the last valid element.
+#pragma once
+#include "rng.hpp"
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace dxmc_dev {
+
+template <typename T>
+struct Consts;
+// the values of std::numbers::{pi, sqrt2, sqrt3, inv_sqrtpi, e}_v<T>
+template <>
+struct Consts<double> {
+    static constexpr double PI = 3.141592653589793238462643383279502884;
+    static constexpr double SQRT2 = 1.414213562373095048801688724209698079;
+    static constexpr double SQRT3 = 1.732050807568877293527446341505872367;
+    static constexpr double INV_SQRTPI = 0.564189583547756286948079451560772586;
+    static constexpr double E = 2.718281828459045235360287471352662498;
+};
+template <>
+struct Consts<float> {
+    static constexpr float PI = 3.141592653589793238462643383279502884f;
+    static constexpr float SQRT2 = 1.414213562373095048801688724209698079f;
+    static constexpr float SQRT3 = 1.732050807568877293527446341505872367f;
+    static constexpr float INV_SQRTPI = 0.564189583547756286948079451560772586f;
+    static constexpr float E = 2.718281828459045235360287471352662498f;
+};
+
+template <typename T>
+struct Particle {
+    T pos[3];
+    T dir[3];
+    T energy;
+    T weight;
+};
+
+template <typename T>
+struct Shell {
+    T binding, electrons, j0, photoProb, yield;
+    T lineProb[3];
+    T lineEnergy[3];
+};
+
+template <typename T>
+struct Heel {
+    T energyStart, energyStep, angleStart, angleStep;
+    uint32_t energySize, angleSize, weightOffset, pad;
+};
+
+// Flat device tables (one copy per precision). For T = float the LUT grid and
+// linear step/energy are in log2 units and each coefficient pair is
+// (log2 mu at the segment's upper node, slope).
+template <typename T>
+struct Tables {
+    uint32_t nMaterials;
+    uint32_t lutNodes;
+    uint32_t lutLinearIndex;
+    uint32_t nSpectra;
+    T lutLinearStep;
+    T lutLinearEnergy;
+    const T* lutGrid;
+    const T* lutCoef; // [mat][node][type][b,a]
+    const T* lutMajorant; // [node][b,a]
+    const T* rita; // [mat][x,e,a,b][56]
+    const T* spline; // [mat][60 + start, step, stop]
+    const Shell<T>* shells; // [mat][12]
+    const uint32_t* specOffset;
+    const uint32_t* specThreshold; // 32-bit truncation of 2^64 mod n per spectrum
+    const T* specProb;
+    const uint32_t* specAlias;
+    const T* specEnergy;
+    const Heel<T>* heel;
+    const T* heelWeight;
+    const uint32_t* bowtieOffset;
+    const T* bowtieAngle;
+    const T* bowtieWeight;
+};
+
+template <typename T>
+struct Exposure {
+    T pos[3];
+    T cosines[6];
+    T beamDir[3];
+    T collimation[4];
+    T weight;
+    T monoEnergy;
+    int32_t spectrum, heel, bowtie, pad;
+};
+
+// Voxel world. FP64 plain path: separate material / density / measurement
+// arrays as the reference; FP32 path: one 8-byte record per voxel
+// (x = density bits, y = material | measurement << 8).
+template <typename T>
+struct World {
+    uint32_t nx, ny, nz, pad;
+    T spacing[3];
+    T extent[6];
+    const uint8_t* material;
+    const double* density;
+    const uint8_t* measurement; // may be null
+    const uint2* voxels;
+};
+
+struct Tally {
+    double* dose;
+    uint32_t* nEvents;
+    double* variance;
+};
+
+// tally plus this thread's event counter
+struct Scorer {
+    Tally tally;
+    uint32_t events;
+};
+
+struct Counters {
+    unsigned long long steps;
+    unsigned long long events;
+};
+
+// Hard bounds that make every loop finish (the reference loops until
+// acceptance): a rejection loop that has not accepted after MAX_TRIALS
+// trials keeps its last trial, a photon still alive after MAX_STEPS Woodcock
+// steps is dropped without scoring. Neither bound is reached by any
+// physical case (acceptance rates are >= 1e-2, histories take < 1e4 steps);
+// the oracle applies the same bounds.
+constexpr uint32_t MAX_TRIALS = 1000;
+constexpr uint32_t MAX_STEPS = 1u << 20;
+
+// ------------------------------------------------------------------ helpers
+
+template <typename T>
+__device__ inline T dmin(T a, T b) { return b < a ? b : a; } // std::min
+template <typename T>
+__device__ inline T dmax(T a, T b) { return a < b ? b : a; } // std::max
+
+// first index i with a[i] > v, n if none (std::upper_bound)
+template <typename T>
+__device__ inline uint32_t upperBound(const T* a, uint32_t n, T v)
+{
+    uint32_t lo = 0, count = n;
+    while (count > 0) {
+        const uint32_t step = count / 2;
+        const uint32_t mid = lo + step;
+        if (!(v < a[mid])) {
+            lo = mid + 1;
+            count -= step + 1;
+        } else
+            count = step;
+    }
+    return lo;
+}
+
+template <typename T>
+__device__ inline T dot3(const T* a, const T* b) { return a[0] * b[0] + a[1] * b[1] + a[2] * b[2]; }
+
+template <typename T>
+__device__ inline void rotate(T* v, const T* axis, T angle)
+{
+    const T s = sin(angle);
+    const T c = cos(angle);
+    const T m = (T(1) - c) * dot3(v, axis);
+    const T r0 = c * v[0] + m * axis[0] + s * (axis[1] * v[2] - axis[2] * v[1]);
+    const T r1 = c * v[1] + m * axis[1] + s * (-axis[0] * v[2] + axis[2] * v[0]);
+    const T r2 = c * v[2] + m * axis[2] + s * (axis[0] * v[1] - axis[1] * v[0]);
+    v[0] = r0;
+    v[1] = r1;
+    v[2] = r2;
+}
+
+template <typename T>
+__device__ inline void peturb(T* dir, T theta, T phi)
+{
+    const T ax = fabs(dir[0]), ay = fabs(dir[1]), az = fabs(dir[2]);
+    const int k = ax <= ay ? (ax <= az ? 0 : 2) : (ay <= az ? 1 : 2);
+    T e[3] = { T(0), T(0), T(0) };
+    e[k] = T(1);
+    T o[3];
+    o[0] = dir[1] * e[2] - dir[2] * e[1];
+    o[1] = dir[2] * e[0] - dir[0] * e[2];
+    o[2] = dir[0] * e[1] - dir[1] * e[0];
+    rotate(o, dir, phi);
+    const T st = sin(theta);
+    const T ct = cos(theta);
+    dir[0] = dir[0] * ct + o[0] * st;
+    dir[1] = dir[1] * ct + o[1] * st;
+    dir[2] = dir[2] * ct + o[2] * st;
+}
+
+// ------------------------------------------------------ attenuation tables
+
+template <typename T>
+struct Lut;
+
+// FP64: mu = 10^(b + a log10 E) exactly as the reference.
+// FP32: the host stores each segment anchored at its upper grid node x_k
+// (log2 units): mu = 2^(y_k + a (log2 E - x_k)), y_k = log2 mu(x_k). This
+// avoids the cancellation of b + a log E in float for steep segments.
+template <>
+struct Lut<double> {
+    __device__ static inline double logE(double E) { return log10(E); }
+    __device__ static inline double value(double b, double a, double le, double) { return pow(10.0, b + a * le); }
+};
+template <>
+struct Lut<float> {
+    __device__ static inline float logE(float E) { return log2f(E); }
+    __device__ static inline float value(float y, float a, float le, float anchor) { return exp2f(y + a * (le - anchor)); }
+};
+
+template <typename T>
+__device__ inline uint32_t lutInterval(const Tables<T>& t, T le)
+{
+    if (le > t.lutLinearEnergy) {
+        const T f = (le - t.lutLinearEnergy) / t.lutLinearStep;
+        if (!(f < T(t.lutNodes))) // beyond the table (or NaN): last interval
+            return t.lutNodes - 1;
+        const uint32_t k = static_cast<uint32_t>(f) + t.lutLinearIndex;
+        return k < t.lutNodes - 1 ? k : t.lutNodes - 1;
+    }
+    const uint32_t up = upperBound(t.lutGrid, t.lutNodes, le);
+    return up != t.lutNodes ? up : t.lutNodes - 1;
+}
+
+// photo, Compton, Rayleigh mass attenuation [cm2/g]
+template <typename T>
+__device__ inline void attenuation(const Tables<T>& t, uint32_t mat, T E, T att[3])
+{
+    const T le = Lut<T>::logE(E);
+    const uint32_t k = lutInterval(t, le);
+    const T anchor = t.lutGrid[k];
+    const T* c = t.lutCoef + (static_cast<size_t>(mat) * t.lutNodes + k) * 6;
+    att[0] = Lut<T>::value(c[0], c[1], le, anchor);
+    att[1] = Lut<T>::value(c[2], c[3], le, anchor);
+    att[2] = Lut<T>::value(c[4], c[5], le, anchor);
+}
+
+template <typename T>
+__device__ inline T majorantInverse(const Tables<T>& t, T E)
+{
+    const T le = Lut<T>::logE(E);
+    const uint32_t k = lutInterval(t, le);
+    const T* c = t.lutMajorant + k * 2;
+    return Lut<T>::value(c[0], c[1], le, t.lutGrid[k]);
+}
+
+// RITA sample of q^2 below qsqrMax (dxmcrandom.hpp:483-500)
+template <typename T>
+__device__ inline T ritaSample(const Tables<T>& t, uint32_t mat, Rng& rng, T maxValue)
+{
+    constexpr uint32_t N = 56;
+    const T* x = t.rita + static_cast<size_t>(mat) * 4 * N;
+    const T* e = x + N;
+    const T* a = x + 2 * N;
+    const T* b = x + 3 * N;
+    const uint32_t im = upperBound(x, N, maxValue);
+    const T cdfMax = im != N ? e[im] : T(1);
+    T res;
+    uint32_t trials = 0;
+    do {
+        const T r = rng.uniform<T>(cdfMax);
+        uint32_t i = upperBound(e, N, r);
+        i = i > 0 ? i - 1 : 0;
+        i = i < N - 2 ? i : N - 2;
+        const T v = r - e[i];
+        const T d = e[i + 1] - e[i];
+        res = x[i] + (1 + a[i] + b[i]) * d * v / (d * d + a[i] * d * v + b[i] * v * v) * (x[i + 1] - x[i]);
+    } while (res > maxValue && ++trials < MAX_TRIALS);
+    return res;
+}
+
+// Compton scatter factor spline S(q)/Z (interpolation.hpp:169-176)
+template <typename T>
+__device__ inline T scatterFactor(const Tables<T>& t, uint32_t mat, T q)
+{
+    const T* s = t.spline + static_cast<size_t>(mat) * 63;
+    const T start = s[60], step = s[61], stop = s[62];
+    const T x = q < start ? start : (stop < q ? stop : q); // std::clamp
+    const uint32_t k = x > start ? static_cast<uint32_t>((x - start) / step) : 0;
+    const uint32_t o = k < 15 ? k * 4 : 14 * 4;
+    return s[o] + s[o + 1] * x + s[o + 2] * x * x + s[o + 3] * x * x * x;
+}
+
+// --------------------------------------------------------------- sources
+
+template <typename T>
+__device__ inline T sampleSpectrum(const Tables<T>& t, int32_t spec, Rng& rng)
+{
+    const uint32_t off = t.specOffset[spec];
+    const uint32_t n = t.specOffset[spec + 1] - off;
+    const T r = rng.uniform<T>();
+    const uint32_t k = rng.index(n, t.specThreshold[spec]);
+    const uint32_t bin = r < t.specProb[off + k] ? k : t.specAlias[off + k];
+    return bin < n - 1 ? rng.uniform<T>(t.specEnergy[off + bin], t.specEnergy[off + bin + 1]) : t.specEnergy[off + bin];
+}
+
+template <typename T>
+__device__ inline T bowtieWeight(const Tables<T>& t, int32_t id, T signedAngle)
+{
+    const T angle = fabs(signedAngle);
+    const uint32_t off = t.bowtieOffset[id];
+    const T* xa = t.bowtieAngle + off;
+    const T* ya = t.bowtieWeight + off;
+    const uint32_t n = t.bowtieOffset[id + 1] - off;
+    uint32_t lo = 0, hi = n - 1;
+    uint32_t it = (hi - lo) / 2;
+    while (it != lo) {
+        if (angle < xa[it]) {
+            hi = it;
+            it = lo;
+        } else
+            lo = it;
+        it += (hi - lo) / 2;
+    }
+    if (angle < xa[lo])
+        return ya[0];
+    if (angle > xa[hi])
+        return ya[n - 1];
+    return ya[lo] + (angle - xa[lo]) * (ya[hi] - ya[lo]) / (xa[hi] - xa[lo]);
+}
+
+template <typename T>
+__device__ inline T heelWeight(const Tables<T>& t, int32_t id, T angle, T energy)
+{
+    const Heel<T> h = t.heel[id];
+    uint32_t e = energy < h.energyStart ? 0 : static_cast<uint32_t>((energy - h.energyStart + T(0.5) * h.energyStep) / h.energyStep);
+    if (e >= h.energySize)
+        e = h.energySize - 1;
+    uint32_t a = angle < h.angleStart ? 0 : static_cast<uint32_t>((angle - h.angleStart) / h.angleStep);
+    if (a >= h.angleSize)
+        a = h.angleSize - 1;
+    const T* w = t.heelWeight + h.weightOffset + e * h.angleSize + a;
+    if (a < h.angleSize - 1) {
+        const T a0 = h.angleStart + h.angleStep * a;
+        const T a1 = h.angleStart + h.angleStep * (a + 1);
+        return w[0] + (w[1] - w[0]) * (angle - a0) / (a1 - a0);
+    }
+    return w[0];
+}
+
+template <typename T>
+__device__ inline void sampleParticle(const Exposure<T>& x, const Tables<T>& t, Rng& rng, Particle<T>& p)
+{
+    const T theta = rng.uniform<T>(x.collimation[0], x.collimation[1]);
+    const T phi = rng.uniform<T>(x.collimation[2], x.collimation[3]);
+    for (int i = 0; i < 3; ++i) {
+        p.pos[i] = x.pos[i];
+        p.dir[i] = x.beamDir[i];
+    }
+    p.weight = x.weight;
+    rotate(p.dir, &x.cosines[3], theta);
+    rotate(p.dir, &x.cosines[0], phi);
+    p.energy = x.spectrum >= 0 ? sampleSpectrum(t, x.spectrum, rng) : x.monoEnergy;
+    if (x.bowtie >= 0)
+        p.weight *= bowtieWeight(t, x.bowtie, theta);
+    if (x.heel >= 0)
+        p.weight *= heelWeight(t, x.heel, phi, p.energy);
+}
+
+// ------------------------------------------------------------------ world
+
+template <typename T>
+__device__ inline bool inside(const World<T>& w, const T* pos)
+{
+    return (pos[0] > w.extent[0] && pos[0] < w.extent[1]) && (pos[1] > w.extent[2] && pos[1] < w.extent[3]) && (pos[2] > w.extent[4] && pos[2] < w.extent[5]);
+}
+
+template <typename T>
+__device__ inline bool transportToWorld(const World<T>& w, Particle<T>& p)
+{
+    if (inside(w, p.pos))
+        return true;
+    // numeric_limits<T>::lowest() / max()
+    T amin = sizeof(T) == 8 ? T(-1.7976931348623157e308) : T(-3.40282347e38f);
+    T amax = sizeof(T) == 8 ? T(1.7976931348623157e308) : T(3.40282347e38f);
+    for (int i = 0; i < 3; ++i) {
+        if (fabs(p.dir[i]) > T(1.0e-9)) {
+            const T a0 = (w.extent[i * 2] - p.pos[i]) / p.dir[i];
+            const T an = (w.extent[i * 2 + 1] - p.pos[i]) / p.dir[i];
+            amin = dmax(amin, dmin(a0, an));
+            amax = dmin(amax, dmax(a0, an));
+        }
+    }
+    if (amin < amax && amin > 0) {
+        for (int i = 0; i < 3; ++i)
+            p.pos[i] += amin * p.dir[i];
+        return true;
+    }
+    return false;
+}
+
+template <typename T>
+__device__ inline uint64_t voxelIndex(const World<T>& w, const T* pos)
+{
+    uint32_t ix = static_cast<uint32_t>((pos[0] - w.extent[0]) / w.spacing[0]);
+    uint32_t iy = static_cast<uint32_t>((pos[1] - w.extent[2]) / w.spacing[1]);
+    uint32_t iz = static_cast<uint32_t>((pos[2] - w.extent[4]) / w.spacing[2]);
+    ix = ix < w.nx ? ix : w.nx - 1;
+    iy = iy < w.ny ? iy : w.ny - 1;
+    iz = iz < w.nz ? iz : w.nz - 1;
+    return (static_cast<uint64_t>(iz) * w.ny + iy) * w.nx + ix;
+}
+
+// safeValueAdd x3 (transport.hpp:208-214): hardware FP64 / u32 atomics
+__device__ inline void score(Scorer& sc, uint64_t idx, double e)
+{
+    unsafeAtomicAdd(sc.tally.dose + idx, e);
+    atomicAdd(sc.tally.nEvents + idx, 1u);
+    unsafeAtomicAdd(sc.tally.variance + idx, e * e);
+    ++sc.events;
+}
+
+// -------------------------------------------------------------- interactions
+
+constexpr double ELECTRON_REST_MASS = 510.9989461;
+constexpr double KEV_TO_ANGSTROM = 12.398520;
+
+template <typename T, int L>
+__device__ inline T photoAbsorption(const Tables<T>& t, Particle<T>& p, uint32_t mat, Rng& rng)
+{
+    const T E = p.energy;
+    p.energy = 0;
+    if constexpr (L < 2) {
+        return E;
+    } else {
+        const Shell<T>* sh = t.shells + mat * 12;
+        T cum[12];
+        T acc = 0;
+        for (int i = 0; i < 12; ++i) {
+            acc = i == 0 ? (E > sh[0].binding ? sh[0].photoProb : T(0)) : acc + (E > sh[i].binding ? sh[i].photoProb : T(0));
+            cum[i] = acc;
+        }
+        int idx = 0;
+        const T sample = cum[11] * rng.uniform<T>();
+        while (cum[idx] < sample && idx < 11)
+            ++idx;
+        if (sh[idx].binding > E || sh[idx].binding < T(1))
+            return E;
+        const T r1 = rng.uniform<T>();
+        if (r1 <= sh[idx].yield) {
+            int line = 0;
+            T r3 = rng.uniform<T>() - sh[idx].lineProb[0];
+            while (r3 > 0 && line < 2) {
+                ++line;
+                r3 -= sh[idx].lineProb[line];
+            }
+            p.energy = sh[idx].lineEnergy[line];
+            const T theta = rng.uniform<T>(Consts<T>::PI);
+            const T phi = rng.uniform<T>(Consts<T>::PI + Consts<T>::PI);
+            peturb(p.dir, theta, phi);
+            return E - p.energy;
+        }
+        return E;
+    }
+}
+
+template <typename T, int L>
+__device__ inline void rayleighScatter(const Tables<T>& t, Particle<T>& p, uint32_t mat, Rng& rng)
+{
+    if constexpr (L == 0) {
+        constexpr T extreme = (4 * Consts<T>::SQRT2) / (3 * Consts<T>::SQRT3);
+        bool reject = true;
+        T theta;
+        uint32_t trials = 0;
+        while (reject && trials++ < MAX_TRIALS) {
+            const T r1 = rng.uniform<T>(T(0), extreme);
+            theta = rng.uniform<T>(T(0), Consts<T>::PI);
+            const T s = sin(theta);
+            reject = r1 > ((2 - s * s) * s);
+        }
+        const T phi = rng.uniform<T>(Consts<T>::PI + Consts<T>::PI);
+        peturb(p.dir, theta, phi);
+    } else {
+        constexpr T kinv = T(1) / T(KEV_TO_ANGSTROM);
+        const T qmax = p.energy * kinv;
+        const T qmax2 = qmax * qmax;
+        T cosAngle;
+        uint32_t trials = 0;
+        do {
+            const T q2 = ritaSample(t, mat, rng, qmax2);
+            const T invE = T(KEV_TO_ANGSTROM) / p.energy;
+            cosAngle = 1 - 2 * q2 * invE * invE;
+        } while ((1 + cosAngle * cosAngle) * T(0.5) < rng.uniform<T>() && ++trials < MAX_TRIALS);
+        const T theta = acos(cosAngle);
+        const T phi = rng.uniform<T>(Consts<T>::PI + Consts<T>::PI);
+        peturb(p.dir, theta, phi);
+    }
+}
+
+template <typename T>
+__device__ inline T comptonImpulse(const Tables<T>& t, Particle<T>& p, uint32_t mat, Rng& rng)
+{
+    const Shell<T>* sh = t.shells + mat * 12;
+    T cum[12];
+    T acc = 0;
+    for (int i = 0; i < 12; ++i) {
+        const T w = (sh[i].binding < p.energy && sh[i].j0 > 0) ? sh[i].electrons : T(0);
+        acc = i == 0 ? w : acc + w;
+        cum[i] = acc;
+    }
+    int si = 0;
+    const T sample = cum[11] * rng.uniform<T>();
+    while (cum[si] < sample && si < 11)
+        ++si;
+    const T U = sh[si].binding / T(ELECTRON_REST_MASS);
+    const T pb = sqrt(2 * U + U * U);
+    const T J0 = sh[si].j0;
+    const T k = p.energy / T(ELECTRON_REST_MASS);
+    if (U > k)
+        return 0;
+    const T emin = 1 / (1 + 2 * k);
+    const T gmaxInv = 1 / (1 / emin + emin);
+    T e, cosAngle;
+    bool rejected;
+    uint32_t trials = 0;
+    do {
+        const T r1 = rng.uniform<T>();
+        e = r1 + (1 - r1) * emin;
+        const T tt = (1 - e) / (k * e);
+        const T sin2 = tt * (2 - tt);
+        cosAngle = 1 - tt;
+        const T g = (1 / e + e - sin2) * gmaxInv;
+        const T r2 = rng.uniform<T>();
+        rejected = r2 > g;
+        if (!rejected) {
+            const T pi = (k * (k - U) * (1 - cosAngle) - U) / sqrt(2 * k * (k - U) * (1 - cosAngle) + U * U);
+            const T kc = k * e;
+            const T qc = sqrt(k * k + kc * kc - 2 * k * kc * cosAngle);
+            const T alpha = qc * (1 + kc * (kc - k * cosAngle) / (qc * qc)) / k;
+            const T bpart = 1 + 2 * J0 * fabs(pi);
+            const T b = (bpart * bpart + 1) / 2;
+            const T expb = exp(-b);
+            T S;
+            if (pi <= -pb) {
+                S = (1 - alpha * pb) * expb / 2;
+            } else if (pi < pb) {
+                constexpr T sp2 = 1 / (Consts<T>::INV_SQRTPI / Consts<T>::SQRT2);
+                const T pabs = fabs(pb);
+                const T piabs = fabs(pi);
+                const T part1 = alpha * sp2 / (4 * J0);
+                constexpr T a1 = T(0.34802);
+                constexpr T a2 = T(-0.0958798);
+                constexpr T a3 = T(0.7478556);
+                const T sqrte = sqrt(Consts<T>::E);
+                const T tp = 1 / (1 + T(0.332673) * (1 + 2 * J0 * pabs));
+                const T tpi = 1 / (1 + T(0.332673) * (1 + 2 * J0 * piabs));
+                const T part2p = sqrte - expb * tp * (a1 + a2 * tp + a3 * tp * tp);
+                const T part2pi = sqrte - expb * tpi * (a1 + a2 * tpi + a3 * tpi * tpi);
+                if (pi <= 0)
+                    S = (1 - alpha * pi) * expb / 2 - part1 * (part2p - part2pi);
+                else
+                    S = 1 - (1 - alpha * pi) * expb / 2 - part1 * (part2p - part2pi);
+            } else {
+                S = 1 - (1 - alpha * pb) * expb / 2;
+            }
+            const T r3 = rng.uniform<T>();
+            rejected = r3 > S;
+            if (!rejected) {
+                T Fmax;
+                if (pi <= -pb)
+                    Fmax = 1 - alpha * pb;
+                else if (pi >= pb)
+                    Fmax = 1 + alpha * pb;
+                else
+                    Fmax = 1 + alpha * pi;
+                const T r4 = rng.uniform<T>();
+                const T rbar2 = 2 * r4 * expb;
+                T pz;
+                if (rbar2 < 1) {
+                    const T part = sqrt(1 - 2 * log(rbar2));
+                    pz = (1 - part) / (2 * J0) / T(ELECTRON_REST_MASS);
+                } else {
+                    const T part = sqrt(1 - 2 * log(2 - rbar2));
+                    pz = (part - 1) / (2 * J0) / T(ELECTRON_REST_MASS);
+                }
+                T Fpz;
+                if (pz <= -pb)
+                    Fpz = 1 - alpha * pb;
+                else if (pz >= pb)
+                    Fpz = 1 + alpha * pb;
+                else
+                    Fpz = 1 + alpha * pz;
+                const T r5 = rng.uniform<T>();
+                rejected = r5 > Fpz / Fmax;
+                if (!rejected) {
+                    const T part = sqrt(1 - 2 * e * cosAngle + e * e * (1 - pz * pz * sin2));
+                    const T kbar = kc / (1 - pz * pz * e * e) * (1 - pz * pz * e * cosAngle + pz * part);
+                    e = kbar / k;
+                }
+            }
+        }
+    } while (rejected && ++trials < MAX_TRIALS);
+    const T theta = acos(cosAngle);
+    const T phi = rng.uniform<T>(Consts<T>::PI + Consts<T>::PI);
+    peturb(p.dir, theta, phi);
+    const T E = p.energy;
+    p.energy *= e;
+    return E - p.energy;
+}
+
+template <typename T, int L>
+__device__ inline T comptonScatter(const Tables<T>& t, Particle<T>& p, uint32_t mat, Rng& rng)
+{
+    if constexpr (L == 2) {
+        return comptonImpulse(t, p, mat, rng);
+    } else {
+        const T E = p.energy;
+        const T k = E / T(ELECTRON_REST_MASS);
+        const T emin = 1 / (1 + 2 * k);
+        const T gmaxInv = 1 / (1 / emin + emin);
+        T e, cosAngle;
+        bool rejected;
+        uint32_t trials = 0;
+        do {
+            const T r1 = rng.uniform<T>();
+            e = r1 + (1 - r1) * emin;
+            const T tt = (1 - e) / (k * e);
+            const T sin2 = tt * (2 - tt);
+            cosAngle = 1 - tt;
+            const T g = (1 / e + e - sin2) * gmaxInv;
+            const T r2 = rng.uniform<T>();
+            if constexpr (L == 1) {
+                constexpr T kinv = T(1) / T(KEV_TO_ANGSTROM);
+                const T q = E * kinv * sqrt(T(0.5) - cosAngle * T(0.5));
+                rejected = r2 > g * scatterFactor(t, mat, q);
+            } else {
+                rejected = r2 > g;
+            }
+        } while (rejected && ++trials < MAX_TRIALS);
+        const T theta = acos(cosAngle);
+        const T phi = rng.uniform<T>(Consts<T>::PI + Consts<T>::PI);
+        peturb(p.dir, theta, phi);
+        p.energy *= e;
+        return E - p.energy;
+    }
+}
+
+// Energy imparted by an event, with the 1 keV cutoff: below it the photon is
+// absorbed and its remaining energy is added (transport.hpp:596-611).
+template <typename T>
+__device__ inline bool depositAfterEvent(Scorer& sc, uint64_t idx, Particle<T>& p, T e, bool& updateMajorant)
+{
+    if (p.energy < T(1)) {
+        const T ei = (e + p.energy) * p.weight;
+        score(sc, idx, static_cast<double>(ei));
+        p.energy = 0;
+        return false;
+    }
+    const T ei = e * p.weight;
+    score(sc, idx, static_cast<double>(ei));
+    updateMajorant = true;
+    return true;
+}
+
+template <typename T, int L>
+__device__ inline bool computeInteractions(const Tables<T>& t, const T att[3], Particle<T>& p, uint32_t mat, Scorer& sc, uint64_t idx,
+    Rng& rng, bool& updateMajorant)
+{
+    const T total = (att[0] + att[1]) + att[2];
+    const T r3 = rng.uniform<T>(total);
+    if (r3 < att[0]) {
+        const T e = photoAbsorption<T, L>(t, p, mat, rng);
+        return depositAfterEvent(sc, idx, p, e, updateMajorant);
+    } else if (r3 < (att[0] + att[1])) {
+        const T e = comptonScatter<T, L>(t, p, mat, rng);
+        return depositAfterEvent(sc, idx, p, e, updateMajorant);
+    }
+    rayleighScatter<T, L>(t, p, mat, rng);
+    return true;
+}
+
+template <typename T, int L>
+__device__ inline bool computeInteractionsForced(const Tables<T>& t, T eventProbability, const T att[3], Particle<T>& p, uint32_t mat,
+    Scorer& sc, uint64_t idx, Rng& rng, bool& updateMajorant)
+{
+    const T total = (att[0] + att[1]) + att[2];
+    const T photoProbability = att[0] / total;
+    const T weightCorrection = eventProbability * photoProbability;
+    {
+        Particle<T> pf = p; // virtual photoabsorption; fluorescence is dropped unless below the cutoff
+        const T ef = photoAbsorption<T, L>(t, pf, mat, rng);
+        T ei;
+        if (pf.energy < T(1))
+            ei = (ef + pf.energy) * pf.weight * weightCorrection;
+        else
+            ei = ef * pf.weight * weightCorrection;
+        score(sc, idx, static_cast<double>(ei));
+    }
+    const T r1 = rng.uniform<T>();
+    if (r1 < eventProbability * (1 - photoProbability)) {
+        const T r2 = rng.uniform<T>(att[1] + att[2]);
+        if (r2 < att[1]) {
+            const T e = comptonScatter<T, L>(t, p, mat, rng);
+            if (!depositAfterEvent(sc, idx, p, e, updateMajorant))
+                return false;
+        } else {
+            rayleighScatter<T, L>(t, p, mat, rng);
+        }
+    }
+    p.weight *= (1 - weightCorrection);
+    return true;
+}
+
+template <typename T>
+__device__ inline void readVoxel(const World<T>& w, uint64_t idx, uint32_t& mat, T& density, uint32_t& meas);
+
+template <>
+__device__ inline void readVoxel<double>(const World<double>& w, uint64_t idx, uint32_t& mat, double& density, uint32_t& meas)
+{
+    mat = w.material[idx];
+    density = w.density[idx];
+    meas = w.measurement ? w.measurement[idx] : 0u;
+}
+template <>
+__device__ inline void readVoxel<float>(const World<float>& w, uint64_t idx, uint32_t& mat, float& density, uint32_t& meas)
+{
+    const uint2 v = w.voxels[idx];
+    density = __uint_as_float(v.x);
+    mat = v.y & 0xffu;
+    meas = (v.y >> 8) & 0xffu;
+}
+
+// One Woodcock delta-tracking step (body of the loop in transport.hpp:650-699):
+// refresh the majorant after an energy change, fly an exponential distance
+// under the majorant, then a real or virtual interaction in the voxel
+// reached, then Russian roulette. Returns false when the photon is gone.
+template <typename T, int L>
+__device__ inline bool woodcockStep(const World<T>& w, const Tables<T>& t, Particle<T>& p, T& majInv, bool& update, Rng& rng, Scorer& sc,
+    uint32_t& steps)
+{
+    if (update) {
+        majInv = majorantInverse(t, p.energy);
+        update = false;
+    }
+    const T r1 = rng.uniform<T>();
+    const T step = -log(r1) * majInv * T(10); // cm -> mm
+    for (int i = 0; i < 3; ++i)
+        p.pos[i] += p.dir[i] * step;
+    if (!inside(w, p.pos))
+        return false;
+    ++steps;
+    const uint64_t idx = voxelIndex(w, p.pos);
+    uint32_t mat, meas;
+    T density;
+    readVoxel(w, idx, mat, density, meas);
+    T att[3];
+    attenuation(t, mat, p.energy, att);
+    const T prob = ((att[0] + att[1]) + att[2]) * density * majInv;
+    bool alive = true;
+    if (meas == 0) {
+        const T r2 = rng.uniform<T>();
+        if (r2 < prob)
+            alive = computeInteractions<T, L>(t, att, p, mat, sc, idx, rng, update);
+    } else {
+        alive = computeInteractionsForced<T, L>(t, prob, att, p, mat, sc, idx, rng, update);
+    }
+    if (alive && p.energy * p.weight < T(5)) { // Russian roulette
+        const T r4 = rng.uniform<T>();
+        if (r4 < T(0.8))
+            alive = false;
+        else
+            p.weight *= T(1) / (T(1) - T(0.8));
+    }
+    return alive;
+}
+
+// Woodcock tracking of one photon until it leaves the world, is absorbed or
+// is killed by Russian roulette (transport.hpp:640-700).
+template <typename T, int L>
+__device__ inline void woodcock(const World<T>& w, const Tables<T>& t, Particle<T>& p, Rng& rng, Scorer& sc, uint32_t& steps)
+{
+    T majInv = 0;
+    bool update = true;
+    for (uint32_t n = 0; n < MAX_STEPS && woodcockStep<T, L>(w, t, p, majInv, update, rng, sc, steps); ++n) { }
+}
+}

@@ -1,0 +1,196 @@
+// Internal definitions shared by the backend's translation units: the opaque
+// C ABI objects, device buffers, error handling and the table builder.
+#pragma once
+#include "device/physics.hpp"
+#include "dxmc_hip.h"
+
+#include <hip/hip_runtime.h>
+#include <string>
+#include <vector>
+
+struct dxmc_ctx {
+    int device = 0;
+    int numCU = 256;
+    std::string arch;
+    hipStream_t stream = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    void* historyCounter = nullptr; // u64 work counter of the persistent kernel
+    std::string error;
+
+    int fail(int status, const std::string& msg)
+    {
+        error = msg;
+        return status;
+    }
+    // records a HIP error, returns non-zero on failure
+    int check(hipError_t e)
+    {
+        if (e == hipSuccess)
+            return 0;
+        error = std::string("HIP error: ") + hipGetErrorString(e);
+        return DXMC_ERR_DEVICE;
+    }
+    template <typename P>
+    int alloc(P** p, size_t bytes)
+    {
+        void* q = nullptr;
+        const hipError_t e = hipMalloc(&q, bytes ? bytes : 1);
+        if (e != hipSuccess) {
+            hipGetLastError();
+            error = "device allocation of " + std::to_string(bytes) + " bytes failed";
+            return DXMC_ERR_MEMORY;
+        }
+        *p = static_cast<P*>(q);
+        return DXMC_OK;
+    }
+    int copy(void* dst, const void* src, size_t bytes)
+    {
+        if (bytes == 0)
+            return DXMC_OK;
+        return check(hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice));
+    }
+};
+
+#define DXMC_CHECK(ctx, expr)                      \
+    do {                                           \
+        if (int dxmc_st_ = (ctx)->check((expr))) \
+            return dxmc_st_;                       \
+    } while (0)
+
+struct dxmc_world {
+    dxmc_ctx* ctx = nullptr;
+    uint64_t n = 0;
+    uint32_t dims[3] = { 0, 0, 0 };
+    double spacing[3] = { 0, 0, 0 };
+    double extent[6] = { 0, 0, 0, 0, 0, 0 };
+    uint32_t nMaterials = 0;
+    bool hasMeasurement = false;
+    std::vector<double> maxDensity; // per material, host copy
+    void* material = nullptr; // u8[n]
+    void* density = nullptr; // f64[n]
+    void* measurement = nullptr; // u8[n] or null
+    void* voxels = nullptr; // uint2[n] FP32 records
+    uint64_t bytes = 0;
+};
+
+struct dxmc_tables {
+    dxmc_ctx* ctx = nullptr;
+    uint32_t nMaterials = 0, nSpectra = 0, nHeel = 0, nBowtie = 0;
+    dxmc_dev::Tables<double> f64 {};
+    dxmc_dev::Tables<float> f32 {};
+    std::vector<void*> allocations;
+};
+
+struct dxmc_tally {
+    dxmc_ctx* ctx = nullptr;
+    uint64_t n = 0;
+    void* dose = nullptr;
+    void* nEvents = nullptr;
+    void* variance = nullptr;
+};
+
+namespace dxmc_backend {
+
+void setGlobalError(const std::string& s);
+const char* globalError();
+
+// RAII device buffer for temporaries
+struct DeviceBuffer {
+    dxmc_ctx* c;
+    void* p = nullptr;
+    explicit DeviceBuffer(dxmc_ctx* ctx)
+        : c(ctx)
+    {
+    }
+    ~DeviceBuffer()
+    {
+        if (p)
+            hipFree(p);
+    }
+    DeviceBuffer(const DeviceBuffer&) = delete;
+    DeviceBuffer& operator=(const DeviceBuffer&) = delete;
+    int alloc(size_t bytes)
+    {
+        if (p) {
+            hipFree(p);
+            p = nullptr;
+        }
+        return c->alloc(&p, bytes);
+    }
+    int upload(const void* src, size_t bytes)
+    {
+        if (int st = alloc(bytes))
+            return st;
+        return c->copy(p, src, bytes);
+    }
+    int zeros(size_t bytes)
+    {
+        if (int st = alloc(bytes))
+            return st;
+        return c->check(hipMemset(p, 0, bytes));
+    }
+};
+
+struct RunBuffers {
+    DeviceBuffer exp64, exp32, startDev, counters;
+    std::vector<uint64_t> start;
+    uint64_t totalHistories = 0;
+    explicit RunBuffers(dxmc_ctx* c)
+        : exp64(c)
+        , exp32(c)
+        , startDev(c)
+        , counters(c)
+    {
+    }
+};
+
+template <typename T>
+inline dxmc_dev::World<T> deviceWorld(const dxmc_world* w)
+{
+    dxmc_dev::World<T> d {};
+    d.nx = w->dims[0];
+    d.ny = w->dims[1];
+    d.nz = w->dims[2];
+    for (int i = 0; i < 3; ++i)
+        d.spacing[i] = static_cast<T>(w->spacing[i]);
+    for (int i = 0; i < 6; ++i)
+        d.extent[i] = static_cast<T>(w->extent[i]);
+    d.material = static_cast<const uint8_t*>(w->material);
+    d.density = static_cast<const double*>(w->density);
+    d.measurement = static_cast<const uint8_t*>(w->measurement);
+    d.voxels = static_cast<const uint2*>(w->voxels);
+    return d;
+}
+
+// block reduction of the step / event counters into the global counters
+__device__ inline void accumulateCounters(dxmc_dev::Counters* ctr, uint32_t steps, uint32_t events)
+{
+    __shared__ unsigned long long s[2];
+    if (threadIdx.x == 0) {
+        s[0] = 0;
+        s[1] = 0;
+    }
+    __syncthreads();
+    if (steps)
+        atomicAdd(&s[0], static_cast<unsigned long long>(steps));
+    if (events)
+        atomicAdd(&s[1], static_cast<unsigned long long>(events));
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        if (s[0])
+            atomicAdd(&ctr->steps, s[0]);
+        if (s[1])
+            atomicAdd(&ctr->events, s[1]);
+    }
+}
+
+bool validateTables(const dxmc_tables_desc& d, std::string& why);
+int buildDeviceTables(dxmc_ctx* c, const dxmc_tables_desc& d, dxmc_tables& t);
+int uploadExposures(dxmc_ctx* c, const dxmc_exposure* x, uint64_t nExp, RunBuffers& rb);
+int runTransport(dxmc_ctx* c, const dxmc_world* w, const dxmc_tables* t, const dxmc_exposure* x, uint64_t nExp, const dxmc_run_cfg& cfg,
+    dxmc_tally* tally, const volatile int32_t* cancel, volatile uint64_t* exposuresDone, dxmc_run_stats* stats);
+// FP32 transport launch (transport_fp32.hip)
+void launchFp32(dxmc_ctx* c, const dxmc_run_cfg& cfg, const dxmc_dev::World<float>& w, const dxmc_dev::Tables<float>& t,
+    const dxmc_dev::Exposure<float>* exps, const uint64_t* expStart, uint32_t nExp, uint64_t hb, uint64_t he, const dxmc_dev::Tally& tally,
+    dxmc_dev::Counters* ctr);
+}

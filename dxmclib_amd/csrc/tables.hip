@@ -1,0 +1,206 @@
+// Host side of the table upload: validation of dxmc_tables_desc and the
+// construction of the FP64 and FP32 device copies.
+//
+// FP32 copy: the attenuation LUT is re-based from log10 to log2 and anchored
+// at each segment's upper grid node, so the kernels evaluate
+// mu = exp2(y_k + a (log2 E - x_k)) with y_k = log2 mu(x_k) (no cancellation
+// for steep segments); grid and linear-index parameters are scaled by
+// log2(10); all other tables are rounded to float. The conversion is done in
+// double on the host.
+#include "internal.hpp"
+
+#include <cmath>
+#include <mutex>
+
+namespace dxmc_backend {
+
+namespace {
+    std::mutex gErrMutex;
+    std::string gErr;
+    constexpr double LOG2_10 = 3.321928094887362347870319429489390175864831393;
+}
+
+void setGlobalError(const std::string& s)
+{
+    std::lock_guard lock(gErrMutex);
+    gErr = s;
+}
+const char* globalError()
+{
+    std::lock_guard lock(gErrMutex);
+    return gErr.c_str();
+}
+
+bool validateTables(const dxmc_tables_desc& d, std::string& why)
+{
+    if (d.n_materials == 0 || d.n_materials > 256)
+        return why = "need 1..256 materials", false;
+    if (d.lut_nodes < 2 || !d.lut_grid || !d.lut_coefficients || !d.lut_majorant)
+        return why = "attenuation LUT missing", false;
+    if (d.lut_linear_index > d.lut_nodes || !(d.lut_linear_step > 0))
+        return why = "LUT linear index parameters invalid", false;
+    for (uint32_t i = 1; i < d.lut_nodes; ++i)
+        if (!(d.lut_grid[i - 1] < d.lut_grid[i]))
+            return why = "LUT grid not increasing", false;
+    if (!d.rita || !d.spline || !d.shells)
+        return why = "RITA, spline or shell tables missing", false;
+    for (uint32_t m = 0; m < d.n_materials; ++m) {
+        const double* s = d.spline + m * 63;
+        if (!(s[61] > 0))
+            return why = "spline step must be positive", false;
+    }
+    if (d.n_spectra) {
+        if (!d.spectrum_offset || !d.spectrum_probability || !d.spectrum_alias || !d.spectrum_energy)
+            return why = "spectrum arrays missing", false;
+        if (d.spectrum_offset[0] != 0)
+            return why = "spectrum offsets must start at 0", false;
+        for (uint32_t s = 0; s < d.n_spectra; ++s) {
+            const uint32_t a = d.spectrum_offset[s], b = d.spectrum_offset[s + 1];
+            if (b <= a)
+                return why = "empty spectrum", false;
+            for (uint32_t i = a; i < b; ++i)
+                if (d.spectrum_alias[i] >= b - a)
+                    return why = "spectrum alias index out of range", false;
+        }
+    }
+    if (d.n_heel) {
+        if (!d.heel || !d.heel_weight)
+            return why = "heel arrays missing", false;
+        for (uint32_t h = 0; h < d.n_heel; ++h)
+            if (d.heel[h].energy_size < 1 || d.heel[h].angle_size < 1 || !(d.heel[h].energy_step > 0) || !(d.heel[h].angle_step > 0))
+                return why = "heel grid invalid", false;
+    }
+    if (d.n_bowtie) {
+        if (!d.bowtie_offset || !d.bowtie_angle || !d.bowtie_weight || d.bowtie_offset[0] != 0)
+            return why = "bowtie arrays missing", false;
+        for (uint32_t b = 0; b < d.n_bowtie; ++b)
+            if (d.bowtie_offset[b + 1] <= d.bowtie_offset[b])
+                return why = "empty bowtie filter", false;
+    }
+    return true;
+}
+
+namespace {
+    template <typename T, typename S>
+    int upload(dxmc_ctx* c, dxmc_tables& t, const std::vector<S>& host, const T** dst)
+    {
+        T* p = nullptr;
+        if (int st = c->alloc(&p, host.size() * sizeof(S)))
+            return st;
+        t.allocations.push_back(p);
+        *dst = p;
+        return c->copy(p, host.data(), host.size() * sizeof(S));
+    }
+    template <typename T>
+    std::vector<T> cast(const double* a, size_t n, double scale = 1.0)
+    {
+        std::vector<T> r(n);
+        for (size_t i = 0; i < n; ++i)
+            r[i] = static_cast<T>(a[i] * scale);
+        return r;
+    }
+
+    template <typename T>
+    int build(dxmc_ctx* c, const dxmc_tables_desc& d, dxmc_tables& t, dxmc_dev::Tables<T>& out, bool base2)
+    {
+        const double ls = base2 ? LOG2_10 : 1.0;
+        out.nMaterials = d.n_materials;
+        out.lutNodes = d.lut_nodes;
+        out.lutLinearIndex = d.lut_linear_index;
+        out.lutLinearStep = static_cast<T>(d.lut_linear_step * ls);
+        out.lutLinearEnergy = static_cast<T>(d.lut_linear_energy * ls);
+        out.nSpectra = d.n_spectra;
+        int st;
+        if ((st = upload(c, t, cast<T>(d.lut_grid, d.lut_nodes, ls), &out.lutGrid)))
+            return st;
+        // FP64: (b, a) as built; FP32: (log2 mu at the upper node x_k, a)
+        auto pair = [&](const double* ba, size_t node, T* out2) {
+            if (base2) {
+                out2[0] = static_cast<T>((ba[0] + ba[1] * d.lut_grid[node]) * ls);
+                out2[1] = static_cast<T>(ba[1]);
+            } else {
+                out2[0] = static_cast<T>(ba[0]);
+                out2[1] = static_cast<T>(ba[1]);
+            }
+        };
+        const size_t nc = size_t(d.n_materials) * d.lut_nodes * 6;
+        std::vector<T> coef(nc);
+        for (size_t i = 0; i < nc; i += 2)
+            pair(d.lut_coefficients + i, (i / 6) % d.lut_nodes, &coef[i]);
+        if ((st = upload(c, t, coef, &out.lutCoef)))
+            return st;
+        std::vector<T> maj(size_t(d.lut_nodes) * 2);
+        for (size_t i = 0; i < maj.size(); i += 2)
+            pair(d.lut_majorant + i, i / 2, &maj[i]);
+        if ((st = upload(c, t, maj, &out.lutMajorant)))
+            return st;
+        if ((st = upload(c, t, cast<T>(d.rita, size_t(d.n_materials) * 4 * 56), &out.rita)))
+            return st;
+        if ((st = upload(c, t, cast<T>(d.spline, size_t(d.n_materials) * 63), &out.spline)))
+            return st;
+        std::vector<dxmc_dev::Shell<T>> sh(size_t(d.n_materials) * 12);
+        for (size_t i = 0; i < sh.size(); ++i) {
+            const dxmc_shell& s = d.shells[i];
+            sh[i].binding = static_cast<T>(s.binding_energy);
+            sh[i].electrons = static_cast<T>(s.number_electrons);
+            sh[i].j0 = static_cast<T>(s.hartree_fock_j0);
+            sh[i].photoProb = static_cast<T>(s.photo_ionization_probability);
+            sh[i].yield = static_cast<T>(s.fluorescence_yield);
+            for (int k = 0; k < 3; ++k) {
+                sh[i].lineProb[k] = static_cast<T>(s.line_probability[k]);
+                sh[i].lineEnergy[k] = static_cast<T>(s.line_energy[k]);
+            }
+        }
+        if ((st = upload(c, t, sh, &out.shells)))
+            return st;
+        // spectra (a dummy entry keeps the pointers valid when there are none)
+        const uint32_t ns = d.n_spectra;
+        std::vector<uint32_t> off(ns + 1, 0), thr(ns ? ns : 1, 0), alias;
+        std::vector<T> prob, energy;
+        for (uint32_t s = 0; s < ns; ++s) {
+            off[s + 1] = d.spectrum_offset[s + 1];
+            const uint64_t n = d.spectrum_offset[s + 1] - d.spectrum_offset[s];
+            thr[s] = static_cast<uint32_t>((0ULL - n) % n);
+        }
+        const uint32_t total = ns ? d.spectrum_offset[ns] : 1;
+        prob = ns ? cast<T>(d.spectrum_probability, total) : std::vector<T>(1, T(1));
+        energy = ns ? cast<T>(d.spectrum_energy, total) : std::vector<T>(1, T(0));
+        alias = ns ? std::vector<uint32_t>(d.spectrum_alias, d.spectrum_alias + total) : std::vector<uint32_t>(1, 0);
+        if ((st = upload(c, t, off, &out.specOffset)) || (st = upload(c, t, thr, &out.specThreshold)) || (st = upload(c, t, prob, &out.specProb))
+            || (st = upload(c, t, alias, &out.specAlias)) || (st = upload(c, t, energy, &out.specEnergy)))
+            return st;
+        std::vector<dxmc_dev::Heel<T>> heel(d.n_heel ? d.n_heel : 1);
+        size_t nhw = 1;
+        for (uint32_t h = 0; h < d.n_heel; ++h) {
+            const dxmc_heel& s = d.heel[h];
+            heel[h] = { static_cast<T>(s.energy_start), static_cast<T>(s.energy_step), static_cast<T>(s.angle_start), static_cast<T>(s.angle_step),
+                s.energy_size, s.angle_size, s.weight_offset, 0 };
+            nhw = std::max<size_t>(nhw, size_t(s.weight_offset) + size_t(s.energy_size) * s.angle_size);
+        }
+        const std::vector<T> hw = d.n_heel ? cast<T>(d.heel_weight, nhw) : std::vector<T>(1, T(1));
+        if ((st = upload(c, t, heel, &out.heel)) || (st = upload(c, t, hw, &out.heelWeight)))
+            return st;
+        const uint32_t nb = d.n_bowtie;
+        std::vector<uint32_t> boff(nb + 1, 0);
+        for (uint32_t b = 0; b <= nb && nb; ++b)
+            boff[b] = d.bowtie_offset[b];
+        const uint32_t btotal = nb ? d.bowtie_offset[nb] : 1;
+        const std::vector<T> ba = nb ? cast<T>(d.bowtie_angle, btotal) : std::vector<T>(1, T(0));
+        const std::vector<T> bw = nb ? cast<T>(d.bowtie_weight, btotal) : std::vector<T>(1, T(1));
+        if ((st = upload(c, t, boff, &out.bowtieOffset)) || (st = upload(c, t, ba, &out.bowtieAngle)) || (st = upload(c, t, bw, &out.bowtieWeight)))
+            return st;
+        return DXMC_OK;
+    }
+}
+
+int buildDeviceTables(dxmc_ctx* c, const dxmc_tables_desc& d, dxmc_tables& t)
+{
+    t.nMaterials = d.n_materials;
+    t.nSpectra = d.n_spectra;
+    t.nHeel = d.n_heel;
+    t.nBowtie = d.n_bowtie;
+    if (int st = build<double>(c, d, t, t.f64, false))
+        return st;
+    return build<float>(c, d, t, t.f32, true);
+}
+}

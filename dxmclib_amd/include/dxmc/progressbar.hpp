@@ -1,0 +1,120 @@
+// Progress, ETA and cancellation of a running transport (reference
+// progressbar.hpp:43-235). The GPU run loop writes the number of finished
+// exposures into exposureCounter() and polls cancelFlag() between kernel
+// launches, so another thread can watch getETA() and call setCancel(true)
+// exactly as with the reference's CPU workers.
+#pragma once
+
+#include "dxmc/floating.hpp"
+
+#include <array>
+#include <chrono>
+#include <cstdint>
+#include <cstdio>
+#include <memory>
+#include <mutex>
+#include <string>
+#include <vector>
+
+namespace dxmc {
+
+template <Floating T>
+struct DoseProgressImageData {
+    std::array<std::size_t, 2> dimensions = { 0, 0 };
+    std::array<T, 2> spacing = { 0, 0 };
+    std::vector<std::uint8_t> image;
+};
+
+template <Floating T>
+class ProgressBar {
+public:
+    enum class Axis { X,
+        Y,
+        Z };
+    ProgressBar() { }
+    ProgressBar(std::uint64_t total) { setTotalExposures(total); }
+    void setTotalExposures(std::uint64_t total)
+    {
+        m_total = total;
+        m_done = 0;
+        m_start = std::chrono::steady_clock::now();
+    }
+    std::uint64_t totalExposures() const { return m_total; }
+    std::uint64_t completedExposures() const { return m_done; }
+    void setPrefixMessage(const std::string& msg) { m_message = msg; }
+    const std::string& prefixMessage() const { return m_message; }
+    void exposureCompleted() { m_done = m_done + 1; }
+    std::string getETA() const
+    {
+        const std::uint64_t done = m_done;
+        if (done == 0)
+            return m_message + "ETA: estimating... [0%]";
+        const double elapsed = std::chrono::duration<double>(std::chrono::steady_clock::now() - m_start).count();
+        const double remaining = elapsed / done * (m_total - done);
+        char buf[128];
+        std::snprintf(buf, sizeof(buf), "ETA: %.0f s [%.0f%%]", remaining, 100.0 * done / (m_total ? m_total : 1));
+        return m_message + buf;
+    }
+    void setCancel(bool c) { m_cancel = c ? 1 : 0; }
+    bool cancel(void) const { return m_cancel != 0; }
+    void setPlaneNormal(Axis a) { m_axis = a; }
+    Axis planeNormal() const { return m_axis; }
+
+    // the device run loop writes / polls these
+    volatile std::uint64_t* exposureCounter() { return &m_done; }
+    const volatile std::int32_t* cancelFlag() const { return &m_cancel; }
+
+    void setDoseData(const T* dose, const std::array<std::size_t, 3>& dims, const std::array<T, 3>& spacing)
+    {
+        std::scoped_lock lock(m_mutex);
+        m_dose = dose;
+        m_dims = dims;
+        m_spacing = spacing;
+    }
+    void clearDoseData()
+    {
+        std::scoped_lock lock(m_mutex);
+        m_dose = nullptr;
+    }
+    // maximum intensity projection of the attached dose along the plane normal
+    std::shared_ptr<DoseProgressImageData<T>> computeDoseProgressImage()
+    {
+        std::scoped_lock lock(m_mutex);
+        if (!m_dose)
+            return nullptr;
+        const int a = m_axis == Axis::X ? 0 : (m_axis == Axis::Y ? 1 : 2);
+        const int u = a == 0 ? 1 : 0, v = a == 2 ? 1 : 2;
+        auto img = std::make_shared<DoseProgressImageData<T>>();
+        img->dimensions = { m_dims[u], m_dims[v] };
+        img->spacing = { m_spacing[u], m_spacing[v] };
+        std::vector<T> mip(m_dims[u] * m_dims[v], T { 0 });
+        T mx = 0;
+        for (std::size_t z = 0; z < m_dims[2]; ++z)
+            for (std::size_t y = 0; y < m_dims[1]; ++y)
+                for (std::size_t x = 0; x < m_dims[0]; ++x) {
+                    const std::size_t idx[3] = { x, y, z };
+                    const T d = m_dose[(z * m_dims[1] + y) * m_dims[0] + x];
+                    T& m = mip[idx[v] * m_dims[u] + idx[u]];
+                    if (d > m)
+                        m = d;
+                    mx = std::max(mx, d);
+                }
+        img->image.resize(mip.size());
+        for (std::size_t i = 0; i < mip.size(); ++i)
+            img->image[i] = mx > 0 ? static_cast<std::uint8_t>(255 * mip[i] / mx) : 0;
+        return img;
+    }
+
+private:
+    volatile std::uint64_t m_total = 0;
+    volatile std::uint64_t m_done = 0;
+    volatile std::int32_t m_cancel = 0;
+    std::chrono::steady_clock::time_point m_start = std::chrono::steady_clock::now();
+    std::string m_message;
+    Axis m_axis = Axis::Z;
+    std::mutex m_mutex;
+    const T* m_dose = nullptr;
+    std::array<std::size_t, 3> m_dims = { 0, 0, 0 };
+    std::array<T, 3> m_spacing = { 0, 0, 0 };
+};
+}

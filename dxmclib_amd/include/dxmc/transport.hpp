@@ -1,0 +1,246 @@
+// Transport: the reference's CPU transport call (transport.hpp:114-840) with
+// the same interface, running the photon histories on an MI355X through the
+// C ABI of include/dxmc_hip.h.
+//
+//   auto result = Transport<double>()(world, &source);   // plain FP64 method
+//   auto result = Transport<float>()(world, &source);    // FP32 persistent path
+//
+// operator() validates the world and source, builds the attenuation tables
+// and the exposure table on the host, uploads them, transports on the device,
+// normalises the FP64 tallies on the device (eV/history or dose) and returns
+// a Result<T>. Invalid inputs and cancellation return the reference's empty
+// result (all zero, numberOfHistories = 0); device errors are reported
+// through lastError() and also return an empty result.
+#pragma once
+
+#include "dxmc/attenuationlut.hpp"
+#include "dxmc/devicetables.hpp"
+#include "dxmc/dxmcrandom.hpp"
+#include "dxmc/exposure.hpp"
+#include "dxmc/floating.hpp"
+#include "dxmc/lowenergycorrectionmodel.hpp"
+#include "dxmc/particle.hpp"
+#include "dxmc/progressbar.hpp"
+#include "dxmc/world.hpp"
+#include "dxmc_hip.h"
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <map>
+#include <memory>
+#include <mutex>
+#include <stdexcept>
+#include <string>
+#include <string_view>
+#include <type_traits>
+#include <vector>
+
+namespace dxmc {
+
+template <Floating T = double>
+struct Result {
+    std::vector<T> dose;
+    std::vector<std::uint32_t> nEvents;
+    std::vector<T> variance;
+    std::uint64_t numberOfHistories { 0 };
+    std::chrono::duration<float> simulationTime { 0 };
+    std::string_view dose_units = "";
+
+    Result() { }
+    Result(std::size_t size)
+        : dose(size, 0)
+        , nEvents(size, 0)
+        , variance(size, 0)
+    {
+    }
+    std::vector<T> relativeError() const
+    {
+        std::vector<T> e(dose.size());
+        for (std::size_t i = 0; i < dose.size(); ++i)
+            e[i] = dose[i] > 0 ? std::sqrt(variance[i]) / dose[i] : 0;
+        return e;
+    }
+};
+
+template <Floating T>
+class Source;
+
+// One HIP context per device, shared by all Transport objects of the process.
+class DeviceContexts {
+public:
+    static dxmc_ctx* get(int device)
+    {
+        static std::mutex mutex;
+        static std::map<int, std::unique_ptr<dxmc_ctx, int (*)(dxmc_ctx*)>> contexts;
+        std::lock_guard lock(mutex);
+        auto it = contexts.find(device);
+        if (it != contexts.end())
+            return it->second.get();
+        dxmc_ctx* c = nullptr;
+        if (dxmc_hip_ctx_create(device, &c) != DXMC_OK)
+            throw std::runtime_error(std::string("dxmc: cannot create a HIP context: ") + dxmc_hip_last_error(nullptr));
+        contexts.emplace(device, std::unique_ptr<dxmc_ctx, int (*)(dxmc_ctx*)>(c, dxmc_hip_ctx_destroy));
+        return c;
+    }
+};
+
+template <Floating T = double>
+class Transport {
+public:
+    enum class OUTPUTMODE {
+        EV_PER_HISTORY,
+        DOSE
+    };
+
+    Transport() = default;
+
+    // kept for interface compatibility: the GPU schedules histories itself
+    void setNumberOfWorkers(std::uint64_t n) { m_nThreads = std::max<std::uint64_t>(n, 1); }
+    std::size_t numberOfWorkers() const { return m_nThreads; }
+    void setLowEnergyCorrectionModel(LOWENERGYCORRECTION m) { m_lowenergyCorrection = m; }
+    LOWENERGYCORRECTION lowEnergyCorrectionModel() const { return m_lowenergyCorrection; }
+    void setOutputMode(OUTPUTMODE m) { m_outputmode = m; }
+    OUTPUTMODE outputMode() const { return m_outputmode; }
+
+    // device-specific settings
+    void setDevice(int d) { m_device = d; }
+    int device() const { return m_device; }
+    void setSeed(std::uint64_t s) { m_seed = s; }
+    std::uint64_t seed() const { return m_seed; }
+    // DXMC_PRECISION_FP64 (default for double) or DXMC_PRECISION_FP32 (default for float)
+    void setDevicePrecision(int p) { m_precision = p; }
+    int devicePrecision() const { return m_precision; }
+    void setRunConfiguration(const dxmc_run_cfg& cfg) { m_tuning = cfg; }
+    const dxmc_run_stats& lastRunStats() const { return m_stats; }
+    const std::string& lastError() const { return m_error; }
+
+    template <typename U>
+        requires std::is_base_of<World<T>, U>::value
+    Result<T> operator()(const U& world, Source<T>* source, ProgressBar<T>* progressbar = nullptr, bool useSourceDoseCalibration = true)
+    {
+        Result<T> result(world.size());
+        m_error.clear();
+        if (!world.isValid() || !source)
+            return result;
+        source->updateFromWorld(world);
+        source->validate();
+        if (!source->isValid())
+            return result;
+        result.numberOfHistories = source->historiesPerExposure() * source->totalExposures();
+
+        // host tables (always FP64; the device derives its FP32 copy)
+        std::vector<double> dens(world.densityArray()->cbegin(), world.densityArray()->cend());
+        const auto maxDens = AttenuationLutInterpolator<double>::maxDensities(world.materialMap().size(), dens.cbegin(), dens.cend(),
+            world.materialIndexArray()->cbegin());
+        m_attenuationLut.generate(world.materialMap(), maxDens, static_cast<double>(source->maxPhotonEnergyProduced()), 1.0);
+
+        TableSet tables;
+        tables.setMaterialTables(m_attenuationLut);
+        ExposureTable<T> exposures(tables);
+        const auto& basis = world.directionCosines();
+        const std::uint64_t nExp = source->totalExposures();
+        for (std::uint64_t i = 0; i < nExp; ++i) {
+            auto e = source->getExposure(i);
+            e.alignToDirectionCosines(basis);
+            exposures.add(e);
+        }
+        if (progressbar) {
+            progressbar->setTotalExposures(nExp);
+            progressbar->setDoseData(result.dose.data(), world.dimensions(), world.spacing());
+        }
+        const auto start = std::chrono::steady_clock::now();
+        std::vector<double> dose(world.size()), variance(world.size());
+        // the calibration factor (a nested transport for CT sources) is applied
+        // by the device normalisation: dose = ei * (calibration / voxel mass)
+        auto calibration = [&]() -> double {
+            if (m_outputmode == OUTPUTMODE::DOSE && useSourceDoseCalibration)
+                return static_cast<double>(source->getCalibrationValue(m_lowenergyCorrection, progressbar));
+            return 1.0;
+        };
+        const bool ok = runOnDevice(world, dens, tables, exposures.records(), progressbar, calibration, dose, result.nEvents, variance,
+            result.numberOfHistories);
+        result.simulationTime = std::chrono::steady_clock::now() - start;
+        if (progressbar)
+            progressbar->clearDoseData();
+        if (!ok || (progressbar && progressbar->cancel())) {
+            std::fill(result.nEvents.begin(), result.nEvents.end(), 0);
+            result.numberOfHistories = 0;
+            return result;
+        }
+        for (std::size_t i = 0; i < dose.size(); ++i) {
+            result.dose[i] = static_cast<T>(dose[i]);
+            result.variance[i] = static_cast<T>(variance[i]);
+        }
+        if (m_outputmode == OUTPUTMODE::DOSE)
+            result.dose_units = useSourceDoseCalibration ? "mGy" : "keV/kg";
+        else
+            result.dose_units = "eV/history";
+        return result;
+    }
+
+    const AttenuationLut<double>& attenuationLut() const { return m_attenuationLut; }
+
+protected:
+    template <typename Calibration>
+    bool runOnDevice(const World<T>& world, const std::vector<double>& dens, const TableSet& tables, const std::vector<dxmc_exposure>& exposures,
+        ProgressBar<T>* pb, Calibration calibration, std::vector<double>& dose, std::vector<std::uint32_t>& nEvents, std::vector<double>& variance,
+        std::uint64_t nHist)
+    {
+        dxmc_ctx* c = DeviceContexts::get(m_device);
+        dxmc_world_desc wd {};
+        for (int i = 0; i < 3; ++i) {
+            wd.dims[i] = world.dimensions()[i];
+            wd.spacing[i] = static_cast<double>(world.spacing()[i]);
+        }
+        for (int i = 0; i < 6; ++i)
+            wd.extent_safe[i] = static_cast<double>(world.matrixExtentSafe()[i]);
+        wd.material = world.materialIndexArray()->data();
+        wd.density_f64 = dens.data();
+        wd.measurement = world.measurementMapArray() ? world.measurementMapArray()->data() : nullptr;
+        wd.n_materials = static_cast<std::uint32_t>(world.materialMap().size());
+        dxmc_world* dw = nullptr;
+        dxmc_tables* dt = nullptr;
+        dxmc_tally* tally = nullptr;
+        auto cleanup = [&]() {
+            if (tally)
+                dxmc_hip_tally_destroy(c, tally);
+            if (dt)
+                dxmc_hip_tables_destroy(c, dt);
+            if (dw)
+                dxmc_hip_world_destroy(c, dw);
+        };
+        const dxmc_tables_desc td = tables.desc();
+        dxmc_run_cfg cfg = m_tuning;
+        cfg.precision = m_precision >= 0 ? m_precision : (std::is_same_v<T, float> ? DXMC_PRECISION_FP32 : DXMC_PRECISION_FP64);
+        cfg.low_energy_correction = static_cast<std::int32_t>(m_lowenergyCorrection);
+        cfg.seed = m_seed;
+        const int mode = m_outputmode == OUTPUTMODE::DOSE ? DXMC_OUTPUT_DOSE : DXMC_OUTPUT_EV_PER_HISTORY;
+        bool ok = !(dxmc_hip_world_upload(c, &wd, &dw) || dxmc_hip_tables_upload(c, &td, &dt) || dxmc_hip_tally_create(c, world.size(), &tally)
+            || dxmc_hip_run(c, dw, dt, exposures.data(), exposures.size(), &cfg, tally, pb ? pb->cancelFlag() : nullptr, pb ? pb->exposureCounter() : nullptr,
+                &m_stats));
+        if (ok && !(pb && pb->cancel()))
+            ok = !((nHist > 1 && dxmc_hip_tally_normalize(c, tally, dw, mode, nHist, mode == DXMC_OUTPUT_DOSE ? calibration() : 1.0))
+                || dxmc_hip_tally_download(c, tally, dose.data(), nEvents.data(), variance.data()));
+        if (!ok) {
+            m_error = dxmc_hip_last_error(c);
+            cleanup();
+            return false;
+        }
+        cleanup();
+        return true;
+    }
+
+private:
+    AttenuationLut<double> m_attenuationLut;
+    std::uint64_t m_nThreads = 1;
+    OUTPUTMODE m_outputmode = OUTPUTMODE::DOSE;
+    LOWENERGYCORRECTION m_lowenergyCorrection = LOWENERGYCORRECTION::LIVERMORE;
+    int m_device = 0;
+    std::uint64_t m_seed = 42;
+    int m_precision = -1;
+    dxmc_run_cfg m_tuning {};
+    dxmc_run_stats m_stats {};
+    std::string m_error;
+};
+}

@@ -1,0 +1,242 @@
+/*
+ * dxmc_hip.h - C ABI of the MI355X (gfx950) photon transport backend.
+ *
+ * This is the drop-in seam for the reference's CPU transport call
+ *   Transport<T>::operator()(const World<T>&, Source<T>*, ProgressBar<T>*, bool)
+ *   (include/dxmc/transport.hpp:135-202)
+ * whose replaceable core is the parallellRun<L>() dispatch
+ *   (transport.hpp:167-173, 745-778) with the table generation before it
+ *   (transport.hpp:157) and the normalisation after it (transport.hpp:187-200).
+ * The reference has no FFI of its own; a C++ host (dxmclib_amd/include/dxmc/
+ * transport.hpp) or any FFI (ctypes, cgo, JNI) binds these entry points.
+ *
+ * Conventions: plain pointers and sizes, no C++ or torch types; every call
+ * returns 0 (DXMC_OK) or a status code, with a message available from
+ * dxmc_hip_last_error(). One host thread per context; contexts are
+ * independent. Device objects (world, tables, tally) belong to the context
+ * that created them and stay resident in HBM until destroyed.
+ */
+#ifndef DXMC_HIP_H
+#define DXMC_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define DXMC_HIP_ABI_VERSION 1
+
+enum dxmc_status {
+    DXMC_OK = 0,
+    DXMC_ERR_ARGUMENT = 1, /* invalid or inconsistent argument */
+    DXMC_ERR_DEVICE = 2, /* HIP runtime error */
+    DXMC_ERR_MEMORY = 3, /* device allocation failed */
+    DXMC_ERR_CANCELLED = 4, /* *cancel became non-zero; tallies hold a partial run */
+    DXMC_ERR_ADMISSION = 5, /* request does not fit the memory budget */
+    DXMC_ERR_STATE = 6 /* object used with the wrong context / destroyed */
+};
+
+enum dxmc_precision {
+    DXMC_PRECISION_FP64 = 0, /* plain method: FP64 throughout, one thread per history */
+    DXMC_PRECISION_FP32 = 1 /* FP32 transport, persistent kernel, FP64 tallies */
+};
+
+enum dxmc_output_mode {
+    DXMC_OUTPUT_EV_PER_HISTORY = 0, /* transport.hpp:780-794 */
+    DXMC_OUTPUT_DOSE = 1 /* transport.hpp:796-816 */
+};
+
+typedef struct dxmc_ctx dxmc_ctx;
+typedef struct dxmc_world dxmc_world;
+typedef struct dxmc_tables dxmc_tables;
+typedef struct dxmc_tally dxmc_tally;
+
+/* ---------------------------------------------------------------- context */
+
+/* Create a context on HIP device 'device'. */
+int dxmc_hip_ctx_create(int device, dxmc_ctx** out);
+int dxmc_hip_ctx_destroy(dxmc_ctx* ctx);
+/* Last error message of the context (or of the last failed ctx_create when
+ * ctx is NULL). The pointer stays valid until the next call on ctx. */
+const char* dxmc_hip_last_error(const dxmc_ctx* ctx);
+int dxmc_hip_abi_version(void);
+/* Device memory in bytes: free and total. */
+int dxmc_hip_mem_info(dxmc_ctx* ctx, uint64_t* free_bytes, uint64_t* total_bytes);
+/* Wait for all work of the context. */
+int dxmc_hip_synchronize(dxmc_ctx* ctx);
+
+/* ------------------------------------------------------------------ world */
+/* Voxel world, reference World<T> (world.hpp:37-227): x fastest layout
+ * index = z*nx*ny + y*nx + x; extent_safe is matrixExtentSafe()
+ * (world.hpp:55-68, nextafter-shrunk extents). */
+typedef struct {
+    uint64_t dims[3];
+    double spacing[3]; /* mm */
+    double extent_safe[6]; /* xmin, xmax, ymin, ymax, zmin, zmax [mm] */
+    const uint8_t* material; /* material index per voxel */
+    const float* density_f32; /* g/cm3 per voxel: exactly one of f32 / f64 */
+    const double* density_f64;
+    const uint8_t* measurement; /* forced-interaction flags; NULL = none set */
+    uint32_t n_materials;
+} dxmc_world_desc;
+
+/* Upload a world; the host arrays may be freed afterwards. */
+int dxmc_hip_world_upload(dxmc_ctx* ctx, const dxmc_world_desc* desc, dxmc_world** out);
+int dxmc_hip_world_destroy(dxmc_ctx* ctx, dxmc_world* world);
+/* Bytes the world occupies on the device. */
+int dxmc_hip_world_bytes(const dxmc_world* world, uint64_t* bytes);
+
+/* ----------------------------------------------------------------- tables */
+/* One grouped electron shell: ElectronShellConfiguration (material.hpp:29-58). */
+typedef struct {
+    double binding_energy; /* keV */
+    double number_electrons; /* normalised electron fraction */
+    double hartree_fock_j0; /* Compton profile at pz = 0 */
+    double photo_ionization_probability;
+    double fluorescence_yield;
+    double line_probability[3];
+    double line_energy[3]; /* keV */
+} dxmc_shell;
+
+/* Heel filter grid: HeelFilter (beamfilters.hpp:435-569). */
+typedef struct {
+    double energy_start, energy_step, angle_start, angle_step;
+    uint32_t energy_size, angle_size;
+    uint32_t weight_offset; /* into heel_weight, energy_size*angle_size values */
+    uint32_t reserved;
+} dxmc_heel;
+
+/* All host-built tables of AttenuationLut (attenuationlut.hpp:42-275) and
+ * of the sources' samplers. */
+typedef struct {
+    uint32_t n_materials;
+    /* log-log attenuation LUT, AttenuationLutInterpolator (attenuationinterpolator.hpp:30-250) */
+    uint32_t lut_nodes; /* grid size after the first node is dropped (= intervals) */
+    const double* lut_grid; /* [lut_nodes] log10(E/keV) */
+    const double* lut_coefficients; /* [n_materials][lut_nodes][photo, compton, rayleigh][b, a] */
+    const double* lut_majorant; /* [lut_nodes][b, a] of log10(1/mu_max) */
+    uint32_t lut_linear_index;
+    double lut_linear_step;
+    double lut_linear_energy;
+    /* Rayleigh form factor sampler RITA<T,56> (dxmcrandom.hpp:333-501) */
+    const double* rita; /* [n_materials][x, e, a, b][56] */
+    /* Compton scatter factor CubicSplineInterpolator<T,16> (interpolation.hpp:84-177) */
+    const double* spline; /* [n_materials][60 coefficients, start, step, stop] */
+    const dxmc_shell* shells; /* [n_materials][12] */
+    /* spectra: SpecterDistribution alias tables (dxmcrandom.hpp:173-330) */
+    uint32_t n_spectra;
+    const uint32_t* spectrum_offset; /* [n_spectra + 1] */
+    const double* spectrum_probability;
+    const uint32_t* spectrum_alias;
+    const double* spectrum_energy;
+    /* heel filters */
+    uint32_t n_heel;
+    const dxmc_heel* heel;
+    const double* heel_weight;
+    /* bowtie filters: sorted |angle| -> normalised weight (beamfilters.hpp:63-198) */
+    uint32_t n_bowtie;
+    const uint32_t* bowtie_offset; /* [n_bowtie + 1] */
+    const double* bowtie_angle;
+    const double* bowtie_weight;
+} dxmc_tables_desc;
+
+int dxmc_hip_tables_upload(dxmc_ctx* ctx, const dxmc_tables_desc* desc, dxmc_tables** out);
+int dxmc_hip_tables_destroy(dxmc_ctx* ctx, dxmc_tables* tables);
+
+/* -------------------------------------------------------------- exposures */
+/* One exposure, already aligned to the world basis
+ * (Exposure::alignToDirectionCosines, exposure.hpp:268-278). Fields follow
+ * Exposure<T> (exposure.hpp:36-49). */
+typedef struct {
+    double position[3]; /* mm */
+    double direction_cosines[6];
+    double beam_direction[3];
+    double collimation_angles[4]; /* x0, x1, y0, y1 [rad] */
+    double weight; /* beam intensity weight */
+    double mono_energy; /* keV, used when spectrum < 0 */
+    uint64_t n_histories;
+    int32_t spectrum; /* index into the spectra of the tables, -1 = monoenergetic */
+    int32_t heel; /* -1 = none */
+    int32_t bowtie; /* -1 = none */
+    int32_t reserved;
+} dxmc_exposure;
+
+/* ------------------------------------------------------------------- tally */
+/* Result<T> (transport.hpp:42-65): dose (energy imparted) f64, event count
+ * u32 and per-event variance sum f64 per voxel. */
+int dxmc_hip_tally_create(dxmc_ctx* ctx, uint64_t n_voxels, dxmc_tally** out);
+int dxmc_hip_tally_destroy(dxmc_ctx* ctx, dxmc_tally* tally);
+int dxmc_hip_tally_reset(dxmc_ctx* ctx, dxmc_tally* tally);
+/* Copy to host; any pointer may be NULL to skip that array. */
+int dxmc_hip_tally_download(dxmc_ctx* ctx, const dxmc_tally* tally, double* dose, uint32_t* n_events, double* variance);
+/* In-place FP64 normalisation on the device:
+ * EV_PER_HISTORY: normalizeScoring (transport.hpp:780-794),
+ * DOSE: energyImpartedToDose with a calibration factor (transport.hpp:796-816). */
+int dxmc_hip_tally_normalize(dxmc_ctx* ctx, dxmc_tally* tally, const dxmc_world* world, int output_mode, uint64_t n_histories,
+    double calibration);
+
+/* -------------------------------------------------------------- transport */
+typedef struct {
+    int32_t precision; /* dxmc_precision */
+    int32_t low_energy_correction; /* 0 NONE, 1 LIVERMORE, 2 IA (lowenergycorrectionmodel.hpp) */
+    uint64_t seed; /* history h uses state1 = 2h+1, state0 = splitmix64(seed ^ splitmix64(h)) */
+    uint64_t first_history; /* global index of the first history of exposure 0 */
+    uint64_t histories_per_launch; /* 0: automatic; bounds cancel/progress latency */
+    uint32_t block_size; /* 0: tuned default */
+    uint32_t blocks_per_cu; /* FP32 persistent kernel occupancy, 0: tuned default */
+    uint32_t regeneration_threshold; /* FP32: dead lanes per wave before refill, 0: tuned */
+    uint32_t flags; /* bit 0: collect step/event counters */
+} dxmc_run_cfg;
+
+typedef struct {
+    uint64_t histories; /* histories started */
+    uint64_t steps; /* Woodcock steps ending inside the world */
+    uint64_t events; /* scoring events (dose/nEvents/variance updates) */
+    double kernel_ms; /* summed transport kernel time (HIP events) */
+    uint64_t launches;
+} dxmc_run_stats;
+
+/* Transport all histories of n_exposures exposures and add their energy
+ * imparted to the tally (not reset). cancel (may be NULL) is polled between
+ * launches: a non-zero value stops the run with DXMC_ERR_CANCELLED.
+ * exposures_done (may be NULL) receives the number of exposures finished.
+ * Replaces parallellRun<L> (transport.hpp:765-778) and its workers. */
+int dxmc_hip_run(dxmc_ctx* ctx, const dxmc_world* world, const dxmc_tables* tables, const dxmc_exposure* exposures,
+    uint64_t n_exposures, const dxmc_run_cfg* cfg, dxmc_tally* tally, const volatile int32_t* cancel,
+    volatile uint64_t* exposures_done, dxmc_run_stats* stats);
+
+/* ------------------------------------------------------ parity entry points */
+/* PCG32 streams (RandomState::pcg32, dxmcrandom.hpp:156-165) of histories
+ * first_history .. first_history+n-1 under 'seed': out[i*draws + k]. */
+int dxmc_hip_rng_streams(dxmc_ctx* ctx, uint64_t seed, uint64_t first_history, uint64_t n, uint32_t draws, uint32_t* out);
+
+/* LUT lookups (attenuationinterpolator.hpp:207-248) at n (material, energy)
+ * pairs in the given precision: out_attenuation[i*3 + type] (photo,
+ * Compton, Rayleigh, cm2/g) and out_majorant_inv[i] (cm). */
+int dxmc_hip_lut_lookup(dxmc_ctx* ctx, const dxmc_tables* tables, int32_t precision, uint64_t n, const uint32_t* material,
+    const double* energy, double* out_attenuation, double* out_majorant_inv);
+
+/* One interaction per sample: kind 0 photoelectric (transport.hpp:216-263),
+ * 1 Compton (:300-483), 2 Rayleigh (:264-298), 3 interaction dispatch
+ * computeInteractions (:583-638), 4 forced dispatch computeInteractionsForced
+ * (:523-581) with event probability 'forced_probability'. Sample i uses the
+ * stream of history first_history+i. particles: [n][pos3, dir3, energy,
+ * weight] in/out; energy_out[i]: energy imparted (kinds 0-2: the function's
+ * return value; 3-4: the tallied value, 0 if none); alive_out[i] (kinds
+ * 3-4): the function's return value. */
+int dxmc_hip_interact(dxmc_ctx* ctx, const dxmc_tables* tables, int32_t precision, int32_t low_energy_correction,
+    int32_t kind, uint64_t n, const uint32_t* material, double forced_probability, uint64_t seed, uint64_t first_history,
+    double* particles, double* energy_out, int32_t* alive_out);
+
+/* Exposure::sampleParticle (exposure.hpp:280-304) followed by
+ * transportParticleToWorld (transport.hpp:702-728) for history
+ * first_history+i of exposure 0: particles [n][8], inside[i]. */
+int dxmc_hip_sample_particles(dxmc_ctx* ctx, const dxmc_world* world, const dxmc_tables* tables, int32_t precision,
+    const dxmc_exposure* exposure, uint64_t seed, uint64_t first_history, uint64_t n, double* particles, int32_t* inside);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
