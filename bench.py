@@ -1,0 +1,224 @@
+"""Throughput benchmark of the photon transport hot path on one MI355X.
+
+A step is one full transport of the workload (every exposure, every
+history) on the FP32 persistent path with the world, tables and tally
+resident in HBM. The default workload is the CT configuration the metric is
+quoted on: 120 kVp CT spiral over a 512x512x300 x 0.8 mm anthropomorphic
+phantom, 3600 exposures, 1e9 histories.
+
+Printed (one JSON line): histories/s of the FP32 path (value), the plain
+FP64 path's histories/s on a fixed subset of the exposures (fp64 block),
+the FP32-vs-FP64 dose agreement on that subset, the roofline of the
+transport kernel (algorithmic bytes 6 per in-world Woodcock step + 40 per
+scoring event, from device counters, over the kernel time measured with HIP
+events on the kernel's stream) and the CPU baseline (the oracle, a CPU
+restatement of the reference method, on the host cores on a bounded sample).
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK = 8.0e12  # MI355X HBM3E bytes/s
+
+
+def parse():
+    p = argparse.ArgumentParser()
+    p.add_argument("--gpus", type=int, default=1)
+    p.add_argument("--steps", type=int, default=3)
+    p.add_argument("--warmup", type=int, default=1)
+    p.add_argument("--config", default="ct_body", choices=["ct_body", "dx_thorax", "water_pencil", "whole_body"])
+    p.add_argument("--fp64-stride", type=int, default=10, help="FP64 path runs every n-th exposure")
+    p.add_argument("--no-fp64", action="store_true")
+    p.add_argument("--no-cpu", action="store_true")
+    p.add_argument("--cpu-seconds", type=float, default=15.0)
+    p.add_argument("--block-size", type=int, default=0)
+    p.add_argument("--blocks-per-cu", type=int, default=0)
+    p.add_argument("--threshold", type=int, default=0)
+    p.add_argument("--low-energy-correction", type=int, default=1)
+    return p.parse_args()
+
+
+def make_workload(name):
+    from dxmclib_amd import workloads
+    if name == "ct_body":
+        return workloads.ct_body()
+    if name == "dx_thorax":
+        return workloads.dx_thorax()
+    if name == "water_pencil":
+        return workloads.water_pencil()
+    return workloads.whole_body()
+
+
+def subset(exposures, stride):
+    from dxmclib_amd import capi
+    idx = list(range(0, len(exposures), stride))
+    out = (capi.Exposure * len(idx))()
+    for j, i in enumerate(idx):
+        out[j] = exposures[i]
+    return out
+
+
+def run_once(dev, w, t, exposures, n, precision, L, tuning):
+    tally = dev.tally(n)
+    try:
+        stats = dev.run(w, t, exposures, tally, precision, L, seed=42, tuning=tuning)
+        dose, _, var = dev.download(tally, n)
+    finally:
+        dev.destroy_tally(tally)
+    return stats, dose, var
+
+
+def cpu_baseline(wl, tables, exposures, ext, L, seconds):
+    """Oracle on the host cores over a bounded sample: every k-th exposure
+    with a reduced history count, sized to about `seconds` of CPU time."""
+    import oracle
+    maxd = [float(wl.density[wl.material == i].max()) if np.any(wl.material == i) else 0.0 for i in range(len(wl.names))]
+    max_e = wl.spec.photon_energy if wl.spec.photon_energy > 0 else wl.spec.tube_voltage
+    orc = oracle.Oracle(wl.names, maxd, max_e, 1.0)
+    src = tables.source_arrays()
+    spectra = []
+    if src.get("spectrum_offset") is not None:
+        spectra = [spectrum_of(wl.spec)]
+    orc.add_source_tables(src, spectra)
+    world = oracle.world_view(wl.material, wl.density.astype(np.float64), wl.spacing, ext)
+    threads = min(os.cpu_count() or 1, 16)
+    stride = max(1, len(exposures) // 36)
+    sample = subset(exposures, stride)
+    # calibrate on a small probe, then size the sample
+    for x in sample:
+        x.n_histories = 2000
+    t0 = time.perf_counter()
+    orc.transport(world, sample, L=L, threads=threads)
+    probe = time.perf_counter() - t0
+    rate = len(sample) * 2000 / max(probe, 1e-3)
+    per = int(max(2000, min(10**6, rate * seconds / len(sample))))
+    for x in sample:
+        x.n_histories = per
+    t0 = time.perf_counter()
+    r = orc.transport(world, sample, L=L, threads=threads)
+    dt = time.perf_counter() - t0
+    return {"value": r["histories"] / dt, "unit": "histories/s", "cores": threads, "kind": "port",
+            "sample": f"{len(sample)} of {len(exposures)} exposures (every {stride}th) x {per} histories, FP64, L={L}, {dt:.1f} s"}
+
+
+def spectrum_of(spec):
+    import ctypes as C
+    from dxmclib_amd import capi
+    n = C.c_uint64()
+    lib = capi.host()
+    lib.dxmc_host_source_spectrum(C.byref(spec), None, None, 0, C.byref(n))
+    e = np.zeros(n.value)
+    w = np.zeros(n.value)
+    lib.dxmc_host_source_spectrum(C.byref(spec), e.ctypes.data_as(C.POINTER(C.c_double)), w.ctypes.data_as(C.POINTER(C.c_double)), n.value,
+                                  C.byref(n))
+    return w, e
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (the transport shards by histories; "
+              "see DESIGN.md for the multi-GPU design)", file=sys.stderr)
+        sys.exit(2)
+    from dxmclib_amd import backend, capi
+
+    t0 = time.perf_counter()
+    wl = make_workload(args.config)
+    ext = backend.world_extent(wl.material.shape[::-1], wl.spacing)
+    maxd = [float(wl.density[wl.material == i].max()) if np.any(wl.material == i) else 0.0 for i in range(len(wl.names))]
+    max_e = wl.spec.photon_energy if wl.spec.type == capi.SOURCE_PENCIL else wl.spec.tube_voltage
+    tables = backend.HostTables(wl.names, maxd, max_e, 1.0)
+    exposures = tables.exposures(wl.spec)
+    setup_s = time.perf_counter() - t0
+    n = wl.voxels
+    L = args.low_energy_correction
+    tuning = capi.RunCfg()
+    tuning.block_size = args.block_size
+    tuning.blocks_per_cu = args.blocks_per_cu
+    tuning.regeneration_threshold = args.threshold
+    tuning.flags = 1
+
+    with backend.Device(0) as dev:
+        w = dev.upload_world(wl.material, wl.density, wl.spacing, ext, len(wl.names))
+        t = dev.upload_tables(tables.desc())
+        for i in range(args.warmup):
+            run_once(dev, w, t, exposures, n, capi.PRECISION_FP32, L, tuning)
+            print(f"warmup {i + 1}/{args.warmup} done", file=sys.stderr, flush=True)
+        dev.synchronize()
+        times, kernel_ms, launches, steps, events = [], 0.0, 0, 0, 0
+        for i in range(args.steps):
+            dev.synchronize()
+            t1 = time.perf_counter()
+            stats, _, _ = run_once(dev, w, t, exposures, n, capi.PRECISION_FP32, L, tuning)
+            dev.synchronize()
+            times.append(time.perf_counter() - t1)
+            kernel_ms += stats.kernel_ms
+            launches += stats.launches
+            steps += stats.steps
+            events += stats.events
+            print(f"step {i + 1}/{args.steps}: {times[-1]:.3f} s, {stats.histories / times[-1]:.4e} histories/s", file=sys.stderr, flush=True)
+        elapsed = sum(times)
+        histories = wl.histories * args.steps
+        value = histories / elapsed
+        alg_bytes = 6.0 * steps + 40.0 * events
+        achieved = alg_bytes / (kernel_ms * 1e-3)
+        result = {
+            "metric": "histories/s at 120kVp CT on 1 GPU (FP32 vs FP64 path); p50/p99 served latency",
+            "value": value,
+            "unit": "histories/s",
+            "n_gpus": 1,
+            "steps": args.steps,
+            "warmup": args.warmup,
+            "ms_per_step": 1e3 * elapsed / args.steps,
+            "higher_is_better": True,
+            "scaling": "weak",
+            "vs_baseline": None,
+            "dtype": "f32 transport, f64 tally",
+            "data": "synthetic",
+            "config": {"workload": wl.name, "histories": wl.histories, "exposures": len(exposures), "voxels": n,
+                       "low_energy_correction": L, "description": wl.description},
+            "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK, "unit": "GB/s", "frac": achieved / HBM_PEAK,
+                         "traffic": None},
+            "counters": {"steps_per_history": steps / histories, "events_per_history": events / histories,
+                         "kernel_ms_per_launch": kernel_ms / max(launches, 1), "launches_per_step": launches / args.steps},
+            "setup_seconds": setup_s,
+        }
+        result["roofline"]["achieved"] = achieved / 1e9
+        result["roofline"]["peak"] = HBM_PEAK / 1e9
+        if not args.no_fp64:
+            sub = subset(exposures, args.fp64_stride)
+            sub_hist = sum(x.n_histories for x in sub)
+            t1 = time.perf_counter()
+            s64, d64, v64 = run_once(dev, w, t, sub, n, capi.PRECISION_FP64, L, tuning)
+            dt64 = time.perf_counter() - t1
+            s32, d32, v32 = run_once(dev, w, t, sub, n, capi.PRECISION_FP32, L, tuning)
+            hot = d64 > 0.1 * d64.max()
+            err = np.abs(d32[hot] - d64[hot]) / d64[hot]
+            sig = np.sqrt(v64[hot]) / d64[hot]
+            result["fp64"] = {"value": sub_hist / dt64, "unit": "histories/s", "histories": sub_hist,
+                              "sample": f"every {args.fp64_stride}th exposure",
+                              "fp32_speedup": value / (sub_hist / dt64),
+                              "fp32_vs_fp64_total_rel": float(abs(d32.sum() - d64.sum()) / d64.sum()),
+                              "fp32_vs_fp64_hot_voxels_mean_rel": float(err.mean()) if err.size else None,
+                              "fp64_hot_voxels_mean_rel_sigma": float(sig.mean()) if sig.size else None}
+            print(f"fp64: {sub_hist / dt64:.4e} histories/s", file=sys.stderr, flush=True)
+        dev.destroy_tables(t)
+        dev.destroy_world(w)
+
+    if not args.no_cpu:
+        result["cpu_baseline"] = cpu_baseline(wl, tables, exposures, ext, L, args.cpu_seconds)
+        print(f"cpu: {result['cpu_baseline']['value']:.4e} histories/s", file=sys.stderr, flush=True)
+    print(json.dumps(result))
+
+
+if __name__ == "__main__":
+    main()

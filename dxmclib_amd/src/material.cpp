@@ -297,7 +297,9 @@ namespace {
     double modelPhotoPartial(int Z, int shell, double E)
     {
         const auto& m = shellModel(Z);
-        if (shell < 0 || shell >= N_SHELLS || m.occupancy[shell] <= 0 || E <= m.binding[shell])
+        // a shell contributes from its edge energy on (the LUT places a node at
+        // the edge and one 1 eV below it, so the jump lies between the two)
+        if (shell < 0 || shell >= N_SHELLS || m.occupancy[shell] <= 0 || E < m.binding[shell])
             return 0;
         return m.photoWeight[shell] * PHOTO_CONSTANT * std::pow(static_cast<double>(Z), PHOTO_Z_EXPONENT) / (E * E * E);
     }

@@ -79,10 +79,14 @@ def test_lut_fp32(device, dev_tables, orc):
     # Rounding the energy to float moves the evaluation point by 2^-24
     # relative; the error is that times the local log-log slope: ~3 for the
     # photoelectric segments (~2e-7) but ~1e4 inside the 1 eV edge intervals.
-    assert np.percentile(err_a, 99.9) <= 2e-6
-    assert np.percentile(err_m, 99.9) <= 2e-6
-    assert err_a.max() <= 1e-4
-    assert err_m.max() <= 1e-4
+    # The first 40 energies sit on absorption edges on purpose: there the
+    # slope of the 1 eV edge interval (~7e3) times 2^-24 bounds the error.
+    assert np.percentile(err_a[40:], 99.9) <= 2e-6
+    assert np.percentile(err_m[40:], 99.9) <= 2e-6
+    assert err_a[40:].max() <= 1e-4
+    assert err_m[40:].max() <= 1e-4
+    assert err_a[:40].max() <= 2e-3
+    assert err_m[:40].max() <= 2e-3
 
 
 def _particles(n, energy, rng):
