@@ -42,7 +42,8 @@ def parse():
     p.add_argument("--cpu-seconds", type=float, default=15.0)
     p.add_argument("--block-size", type=int, default=0)
     p.add_argument("--blocks-per-cu", type=int, default=0)
-    p.add_argument("--threshold", type=int, default=0)
+    p.add_argument("--threshold", type=int, default=0, help="FP32 regeneration threshold (dead lanes per wave), 0: default")
+    p.add_argument("--interaction-threshold", type=int, default=0, help="FP32 pending interactions per wave before they run, 0: default")
     p.add_argument("--low-energy-correction", type=int, default=1)
     return p.parse_args()
 
@@ -145,6 +146,7 @@ def main():
     tuning.block_size = args.block_size
     tuning.blocks_per_cu = args.blocks_per_cu
     tuning.regeneration_threshold = args.threshold
+    tuning.interaction_threshold = args.interaction_threshold
     tuning.flags = 1
 
     with backend.Device(0) as dev:

@@ -133,6 +133,8 @@ class RunCfg(C.Structure):
         ("blocks_per_cu", C.c_uint32),
         ("regeneration_threshold", C.c_uint32),
         ("flags", C.c_uint32),
+        ("interaction_threshold", C.c_uint32),
+        ("reserved", C.c_uint32),
     ]
 
 

@@ -1,53 +1,95 @@
-// FP32 photon transport: persistent kernel with wave-level regeneration.
+// FP32 photon transport: persistent kernel with wave-level regeneration and
+// interaction compaction.
 //
 // Each lane carries one photon (position, direction, energy, weight,
-// majorant, PCG state) in registers and advances it one Woodcock step per
-// loop iteration. Lanes whose photon has died wait until at least
-// 'regeneration_threshold' lanes of the wave are dead (or none is alive);
-// then one lane claims that many history indices with a single atomic on a
-// global counter and every dead lane starts the history at its rank among the
-// dead lanes (ballot + mbcnt). Histories keep their own PCG stream (the stream
-// depends only on the global history index), so results do not depend on
-// which lane or wave ran a history, on the grid size or on the threshold.
-// The grid is sized to the machine (CUs x blocks per CU) and drains when the
-// counter passes the end of the launch range and no lane is alive.
+// majorant, PCG state) in registers and is in one of three phases:
+//   DEAD     no photon: waits for regeneration
+//   FLYING   advances one Woodcock flight per loop iteration (distance under
+//            the majorant, voxel gather, real-or-virtual decision, roulette
+//            after virtual collisions)
+//   PENDING  a real interaction is due in the voxel just reached
+// Per iteration the wave does one of two things, decided from ballots:
+//   - when at least 'interaction_threshold' lanes are PENDING (or none is
+//     FLYING) it runs the interaction code for the pending lanes only,
+//   - otherwise it advances the FLYING lanes.
+// So the long, branchy interaction samplers (Compton/Rayleigh rejection
+// loops, fluorescence) run with many lanes active instead of with the few
+// lanes that happen to interact in a given flight, while the cheap flights
+// run nearly full. Dead lanes are refilled when at least
+// 'regeneration_threshold' of them wait (or the whole wave is dead): one lane
+// claims that many history indices with a single atomic, each dead lane takes
+// its rank among the dead lanes (ballot + mbcnt).
 //
-// Arithmetic is FP32 (LUT in base 2, FP32 transcendentals); each scoring
-// event is flushed to the FP64 tallies with hardware FP64 atomics.
+// Each history keeps its own PCG stream and consumes it in the reference's
+// order (flight, decision, interaction, roulette), so delaying an interaction
+// changes nothing but timing: results depend neither on the lane or wave that
+// ran a history nor on the grid size or thresholds.
+//
+// Arithmetic is FP32 (base-2 anchored LUT, FP32 transcendentals, voxel index
+// by multiplication with the reciprocal spacing); each scoring event is added
+// to the FP64 tallies with hardware FP64 atomics.
 #include "internal.hpp"
 
 using namespace dxmc_dev;
 
 namespace {
 
-__device__ inline uint32_t laneId() { return __lane_id(); }
+enum : uint32_t { DEAD = 0, FLYING = 1, PENDING = 2 };
 
 __device__ inline uint32_t rankBelow(uint64_t mask)
 {
     return __builtin_amdgcn_mbcnt_hi(static_cast<uint32_t>(mask >> 32), __builtin_amdgcn_mbcnt_lo(static_cast<uint32_t>(mask), 0u));
 }
 
+// voxel index with the reciprocal spacing (FP32 path), kept inside the grid
+__device__ inline uint64_t voxelIndexFast(const World<float>& w, const float* pos, const float* inv)
+{
+    uint32_t ix = static_cast<uint32_t>((pos[0] - w.extent[0]) * inv[0]);
+    uint32_t iy = static_cast<uint32_t>((pos[1] - w.extent[2]) * inv[1]);
+    uint32_t iz = static_cast<uint32_t>((pos[2] - w.extent[4]) * inv[2]);
+    ix = ix < w.nx ? ix : w.nx - 1;
+    iy = iy < w.ny ? iy : w.ny - 1;
+    iz = iz < w.nz ? iz : w.nz - 1;
+    return (static_cast<uint64_t>(iz) * w.ny + iy) * w.nx + ix;
+}
+
+// Russian roulette after a step that left the photon alive (transport.hpp:684-694)
+__device__ inline bool roulette(Particle<float>& p, Rng& rng)
+{
+    if (p.energy * p.weight < 5.0f) {
+        if (rng.uniform<float>() < 0.8f)
+            return false;
+        p.weight *= 1.0f / (1.0f - 0.8f);
+    }
+    return true;
+}
+
 template <int L>
 __global__ __launch_bounds__(256) void transportPersistent(World<float> w, Tables<float> t, const Exposure<float>* exps, const uint64_t* expStart,
-    uint32_t nExp, uint64_t hBegin, uint64_t hEnd, uint64_t seed, uint64_t firstHistory, Tally tally, Counters* ctr,
-    unsigned long long* next, uint32_t threshold)
+    uint32_t nExp, uint64_t hBegin, uint64_t hEnd, uint64_t seed, uint64_t firstHistory, Tally tally, Counters* ctr, unsigned long long* next,
+    uint32_t regenThreshold, uint32_t interactThreshold)
 {
     Particle<float> p;
     Rng rng { 0, 1 };
     float majInv = 0;
     bool update = true;
-    bool alive = false;
+    uint32_t phase = DEAD;
     bool exhausted = false;
     Scorer sc { tally, 0 };
     uint32_t steps = 0;
     uint32_t historySteps = 0;
-    const uint32_t lane = laneId();
+    // the pending interaction
+    uint64_t idx = 0;
+    uint32_t mat = 0, meas = 0;
+    float att[3] = { 0, 0, 0 };
+    float prob = 0;
+    const float inv[3] = { 1.0f / w.spacing[0], 1.0f / w.spacing[1], 1.0f / w.spacing[2] };
+    const uint32_t lane = __lane_id();
 
     for (;;) {
-        const uint64_t dead = __ballot(!alive);
+        const uint64_t dead = __ballot(phase == DEAD);
         const uint32_t nDead = __popcll(dead);
-        if (!exhausted && nDead > 0 && (nDead >= threshold || nDead == 64u)) {
-            // claim nDead histories for the dead lanes of this wave
+        if (!exhausted && nDead > 0 && (nDead >= regenThreshold || nDead == 64u)) {
             const uint32_t leader = __ffsll(static_cast<unsigned long long>(dead)) - 1;
             unsigned long long base = 0;
             if (lane == leader)
@@ -55,32 +97,62 @@ __global__ __launch_bounds__(256) void transportPersistent(World<float> w, Table
             base = __shfl(base, static_cast<int>(leader));
             if (hBegin + base + nDead >= hEnd)
                 exhausted = true;
-            if (!alive) {
+            if (phase == DEAD) {
                 const uint64_t h = hBegin + base + rankBelow(dead);
                 if (h < hEnd) {
                     const uint32_t e = upperBound(expStart, nExp + 1, h) - 1;
                     rng = Rng::forHistory(seed, firstHistory + h);
                     sampleParticle(exps[e], t, rng, p);
-                    alive = transportToWorld(w, p);
+                    phase = transportToWorld(w, p) ? FLYING : DEAD;
                     update = true;
                     historySteps = 0;
                 }
             }
         }
-        if (!__any(alive)) {
+        const uint64_t pend = __ballot(phase == PENDING);
+        const uint64_t fly = __ballot(phase == FLYING);
+        if ((pend | fly) == 0) {
             if (exhausted)
                 break;
             continue;
         }
-        if (alive) {
-            if (L == 0)
-                alive = woodcockStep<float, 0>(w, t, p, majInv, update, rng, sc, steps);
-            else if (L == 1)
-                alive = woodcockStep<float, 1>(w, t, p, majInv, update, rng, sc, steps);
-            else
-                alive = woodcockStep<float, 2>(w, t, p, majInv, update, rng, sc, steps);
-            if (++historySteps >= MAX_STEPS)
-                alive = false;
+        const uint32_t nPend = __popcll(pend);
+        if (nPend != 0 && (nPend >= interactThreshold || fly == 0)) {
+            if (phase == PENDING) {
+                bool alive = meas == 0 ? computeInteractions<float, L>(t, att, p, mat, sc, idx, rng, update)
+                                       : computeInteractionsForced<float, L>(t, prob, att, p, mat, sc, idx, rng, update);
+                if (alive)
+                    alive = roulette(p, rng);
+                phase = alive ? FLYING : DEAD;
+            }
+        } else if (phase == FLYING) {
+            if (update) {
+                majInv = majorantInverse(t, p.energy);
+                update = false;
+            }
+            const float step = -logf(rng.uniform<float>()) * majInv * 10.0f; // cm -> mm
+            for (int i = 0; i < 3; ++i)
+                p.pos[i] += p.dir[i] * step;
+            if (!inside(w, p.pos)) {
+                phase = DEAD;
+            } else {
+                ++steps;
+                idx = voxelIndexFast(w, p.pos, inv);
+                float density;
+                readVoxel(w, idx, mat, density, meas);
+                attenuation(t, mat, p.energy, att);
+                prob = ((att[0] + att[1]) + att[2]) * density * majInv;
+                if (meas == 0) {
+                    if (rng.uniform<float>() < prob)
+                        phase = PENDING;
+                    else if (!roulette(p, rng))
+                        phase = DEAD;
+                } else {
+                    phase = PENDING;
+                }
+                if (phase != DEAD && ++historySteps >= MAX_STEPS)
+                    phase = DEAD;
+            }
         }
     }
     if (ctr)
@@ -97,19 +169,31 @@ void launchFp32(dxmc_ctx* c, const dxmc_run_cfg& cfg, const World<float>& w, con
     auto* nx = static_cast<unsigned long long*>(c->historyCounter);
     hipMemsetAsync(nx, 0, sizeof(unsigned long long), c->stream);
     const uint32_t block = 256;
-    const uint32_t perCU = cfg.blocks_per_cu ? cfg.blocks_per_cu : 8;
-    const uint32_t threshold = cfg.regeneration_threshold ? std::min<uint32_t>(cfg.regeneration_threshold, 64u) : 16u;
+    // default: as many blocks as stay resident on a CU (register-limited)
+    uint32_t perCU = cfg.blocks_per_cu;
+    if (perCU == 0) {
+        int resident = 0;
+        const void* k = cfg.low_energy_correction == 0 ? reinterpret_cast<const void*>(&transportPersistent<0>)
+            : cfg.low_energy_correction == 1           ? reinterpret_cast<const void*>(&transportPersistent<1>)
+                                                       : reinterpret_cast<const void*>(&transportPersistent<2>);
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&resident, k, static_cast<int>(block), 0) != hipSuccess || resident <= 0)
+            resident = 4;
+        perCU = static_cast<uint32_t>(resident);
+    }
+    const uint32_t regen = cfg.regeneration_threshold ? std::min<uint32_t>(cfg.regeneration_threshold, 64u) : 16u;
+    const uint32_t interact = cfg.interaction_threshold ? std::min<uint32_t>(cfg.interaction_threshold, 64u) : 24u;
     uint64_t grid = static_cast<uint64_t>(c->numCU) * perCU;
     const uint64_t lanesNeeded = (he - hb + 63) / 64 * 64;
     grid = std::max<uint64_t>(1, std::min<uint64_t>(grid, (lanesNeeded + block - 1) / block));
+    const dim3 g(static_cast<uint32_t>(grid)), b(block);
     if (cfg.low_energy_correction == 0)
-        hipLaunchKernelGGL(transportPersistent<0>, dim3(static_cast<uint32_t>(grid)), dim3(block), 0, c->stream, w, t, exps, expStart, nExp, hb, he,
-            cfg.seed, cfg.first_history, tally, ctr, nx, threshold);
+        hipLaunchKernelGGL(transportPersistent<0>, g, b, 0, c->stream, w, t, exps, expStart, nExp, hb, he, cfg.seed, cfg.first_history, tally, ctr, nx,
+            regen, interact);
     else if (cfg.low_energy_correction == 1)
-        hipLaunchKernelGGL(transportPersistent<1>, dim3(static_cast<uint32_t>(grid)), dim3(block), 0, c->stream, w, t, exps, expStart, nExp, hb, he,
-            cfg.seed, cfg.first_history, tally, ctr, nx, threshold);
+        hipLaunchKernelGGL(transportPersistent<1>, g, b, 0, c->stream, w, t, exps, expStart, nExp, hb, he, cfg.seed, cfg.first_history, tally, ctr, nx,
+            regen, interact);
     else
-        hipLaunchKernelGGL(transportPersistent<2>, dim3(static_cast<uint32_t>(grid)), dim3(block), 0, c->stream, w, t, exps, expStart, nExp, hb, he,
-            cfg.seed, cfg.first_history, tally, ctr, nx, threshold);
+        hipLaunchKernelGGL(transportPersistent<2>, g, b, 0, c->stream, w, t, exps, expStart, nExp, hb, he, cfg.seed, cfg.first_history, tally, ctr, nx,
+            regen, interact);
 }
 }

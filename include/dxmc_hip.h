@@ -188,6 +188,8 @@ typedef struct {
     uint32_t blocks_per_cu; /* FP32 persistent kernel occupancy, 0: tuned default */
     uint32_t regeneration_threshold; /* FP32: dead lanes per wave before refill, 0: tuned */
     uint32_t flags; /* bit 0: collect step/event counters */
+    uint32_t interaction_threshold; /* FP32: pending interactions per wave before they run, 0: tuned */
+    uint32_t reserved; /* must be 0 */
 } dxmc_run_cfg;
 
 typedef struct {
