@@ -69,6 +69,7 @@ def subset(exposures, stride):
 
 
 def run_once(dev, w, t, exposures, n, precision, L, tuning):
+    """Transport into a fresh tally and download it (parity comparisons)."""
     tally = dev.tally(n)
     try:
         stats = dev.run(w, t, exposures, tally, precision, L, seed=42, tuning=tuning)
@@ -76,6 +77,17 @@ def run_once(dev, w, t, exposures, n, precision, L, tuning):
     finally:
         dev.destroy_tally(tally)
     return stats, dose, var
+
+
+def device_step(dev, w, t, tally, exposures, histories, precision, L, tuning):
+    """One job on the device: clear the tally, transport every history,
+    normalise to dose. Inputs and the result stay resident in HBM."""
+    from dxmclib_amd import capi
+    dev.reset_tally(tally)
+    stats = dev.run(w, t, exposures, tally, precision, L, seed=42, tuning=tuning)
+    dev.normalize(tally, w, capi.OUTPUT_DOSE, histories)
+    dev.synchronize()
+    return stats
 
 
 def cpu_baseline(wl, tables, exposures, ext, L, seconds):
@@ -152,22 +164,27 @@ def main():
     with backend.Device(0) as dev:
         w = dev.upload_world(wl.material, wl.density, wl.spacing, ext, len(wl.names))
         t = dev.upload_tables(tables.desc())
+        tally = dev.tally(n)
         for i in range(args.warmup):
-            run_once(dev, w, t, exposures, n, capi.PRECISION_FP32, L, tuning)
+            device_step(dev, w, t, tally, exposures, wl.histories, capi.PRECISION_FP32, L, tuning)
             print(f"warmup {i + 1}/{args.warmup} done", file=sys.stderr, flush=True)
         dev.synchronize()
         times, kernel_ms, launches, steps, events = [], 0.0, 0, 0, 0
         for i in range(args.steps):
             dev.synchronize()
             t1 = time.perf_counter()
-            stats, _, _ = run_once(dev, w, t, exposures, n, capi.PRECISION_FP32, L, tuning)
-            dev.synchronize()
+            stats = device_step(dev, w, t, tally, exposures, wl.histories, capi.PRECISION_FP32, L, tuning)
             times.append(time.perf_counter() - t1)
             kernel_ms += stats.kernel_ms
             launches += stats.launches
             steps += stats.steps
             events += stats.events
             print(f"step {i + 1}/{args.steps}: {times[-1]:.3f} s, {stats.histories / times[-1]:.4e} histories/s", file=sys.stderr, flush=True)
+        # the result handed to a host caller crosses PCIe: reported beside value
+        t1 = time.perf_counter()
+        dev.download(tally, n)
+        download_s = time.perf_counter() - t1
+        dev.destroy_tally(tally)
         elapsed = sum(times)
         histories = wl.histories * args.steps
         value = histories / elapsed
@@ -193,15 +210,21 @@ def main():
             "counters": {"steps_per_history": steps / histories, "events_per_history": events / histories,
                          "kernel_ms_per_launch": kernel_ms / max(launches, 1), "launches_per_step": launches / args.steps},
             "setup_seconds": setup_s,
+            "host_download_seconds": download_s,
+            "pcie_inclusive_value": wl.histories / (sum(times) / args.steps + download_s),
         }
         result["roofline"]["achieved"] = achieved / 1e9
         result["roofline"]["peak"] = HBM_PEAK / 1e9
         if not args.no_fp64:
             sub = subset(exposures, args.fp64_stride)
             sub_hist = sum(x.n_histories for x in sub)
+            tally = dev.tally(n)
             t1 = time.perf_counter()
-            s64, d64, v64 = run_once(dev, w, t, sub, n, capi.PRECISION_FP64, L, tuning)
+            device_step(dev, w, t, tally, sub, sub_hist, capi.PRECISION_FP64, L, tuning)
             dt64 = time.perf_counter() - t1
+            dev.destroy_tally(tally)
+            # dose maps of both paths on identical streams
+            s64, d64, v64 = run_once(dev, w, t, sub, n, capi.PRECISION_FP64, L, tuning)
             s32, d32, v32 = run_once(dev, w, t, sub, n, capi.PRECISION_FP32, L, tuning)
             hot = d64 > 0.1 * d64.max()
             err = np.abs(d32[hot] - d64[hot]) / d64[hot]
@@ -210,8 +233,12 @@ def main():
                               "sample": f"every {args.fp64_stride}th exposure",
                               "fp32_speedup": value / (sub_hist / dt64),
                               "fp32_vs_fp64_total_rel": float(abs(d32.sum() - d64.sum()) / d64.sum()),
-                              "fp32_vs_fp64_hot_voxels_mean_rel": float(err.mean()) if err.size else None,
-                              "fp64_hot_voxels_mean_rel_sigma": float(sig.mean()) if sig.size else None}
+                              "hot_voxels": int(hot.sum()),
+                              "fp32_vs_fp64_hot_rel_mean": float(err.mean()) if err.size else None,
+                              "fp32_vs_fp64_hot_rel_p99": float(np.percentile(err, 99)) if err.size else None,
+                              "fp32_vs_fp64_hot_rel_max": float(err.max()) if err.size else None,
+                              "fp32_vs_fp64_hot_frac_below_1e-3": float(np.mean(err < 1e-3)) if err.size else None,
+                              "fp64_hot_rel_sigma_mean": float(sig.mean()) if sig.size else None}
             print(f"fp64: {sub_hist / dt64:.4e} histories/s", file=sys.stderr, flush=True)
         dev.destroy_tables(t)
         dev.destroy_world(w)

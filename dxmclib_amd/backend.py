@@ -208,6 +208,11 @@ class Device:
         self._check(self.lib.dxmc_hip_world_upload(self.ctx, C.byref(d), C.byref(h)))
         return h
 
+    def world_bytes(self, h):
+        b = C.c_uint64()
+        self._check(self.lib.dxmc_hip_world_bytes(h, C.byref(b)))
+        return b.value
+
     def destroy_world(self, h):
         self._check(self.lib.dxmc_hip_world_destroy(self.ctx, h))
 

@@ -112,17 +112,21 @@ struct Exposure {
 };
 
 // Voxel world. FP64 plain path: separate material / density / measurement
-// arrays as the reference; FP32 path: one 8-byte record per voxel
-// (x = density bits, y = material | measurement << 8).
+// arrays as the reference. FP32 path: a record per distinct voxel content
+// (x = FP32 density bits, y = material | measurement << 8) in 'palette' and
+// a 1- or 2-byte palette index per voxel in 'voxels' (voxelBytes 1 or 2);
+// worlds with more than 65536 distinct records keep the 8-byte record itself
+// per voxel (voxelBytes 8, palette unused).
 template <typename T>
 struct World {
-    uint32_t nx, ny, nz, pad;
+    uint32_t nx, ny, nz, voxelBytes;
     T spacing[3];
     T extent[6];
     const uint8_t* material;
     const double* density;
     const uint8_t* measurement; // may be null
-    const uint2* voxels;
+    const void* voxels;
+    const uint2* palette;
 };
 
 struct Tally {
@@ -737,7 +741,13 @@ __device__ inline void readVoxel<double>(const World<double>& w, uint64_t idx, u
 template <>
 __device__ inline void readVoxel<float>(const World<float>& w, uint64_t idx, uint32_t& mat, float& density, uint32_t& meas)
 {
-    const uint2 v = w.voxels[idx];
+    uint2 v;
+    if (w.voxelBytes == 1)
+        v = w.palette[static_cast<const uint8_t*>(w.voxels)[idx]];
+    else if (w.voxelBytes == 2)
+        v = w.palette[static_cast<const uint16_t*>(w.voxels)[idx]];
+    else
+        v = static_cast<const uint2*>(w.voxels)[idx];
     density = __uint_as_float(v.x);
     mat = v.y & 0xffu;
     meas = (v.y >> 8) & 0xffu;

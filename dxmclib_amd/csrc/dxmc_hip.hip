@@ -17,6 +17,7 @@
 #include "internal.hpp"
 
 #include <algorithm>
+#include <bit>
 #include <chrono>
 #include <cmath>
 #include <cstring>
@@ -140,8 +141,7 @@ __global__ void sampleParticles(World<T> w, Tables<T> t, Exposure<T> x, uint64_t
 }
 
 // normalizeScoring (transport.hpp:780-794) / energyImpartedToDose (:796-816)
-__global__ void normalizeTally(Tally tally, uint64_t n, int mode, uint64_t nHist, double voxelVolume, double calibration,
-    const double* dens64, const uint2* vox32)
+__global__ void normalizeTally(Tally tally, uint64_t n, int mode, uint64_t nHist, double voxelVolume, double calibration, const double* dens64)
 {
     const uint64_t i = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
     if (i >= n)
@@ -154,7 +154,7 @@ __global__ void normalizeTally(Tally tally, uint64_t n, int mode, uint64_t nHist
         tally.dose[i] = d;
         tally.variance[i] = (tally.variance[i] * hv - d * d) * hInv;
     } else {
-        const double de = dens64 ? dens64[i] : static_cast<double>(__uint_as_float(vox32[i].x));
+        const double de = dens64[i];
         const double mass = de * voxelVolume * 0.001;
         const double factor = calibration / mass;
         tally.dose[i] = de > 0.0 ? tally.dose[i] * factor : 0.0;
@@ -181,6 +181,46 @@ __global__ void packVoxels(const uint8_t* mat, const double* dens, const uint8_t
         v.y = static_cast<uint32_t>(mat[i]) | (meas ? static_cast<uint32_t>(meas[i]) << 8 : 0u);
         vox[i] = v;
     }
+}
+
+// Distinct FP32 voxel records (density bits, material | measurement << 8)
+// in order of first appearance, and the palette index of every voxel.
+// Returns false (palette unusable) beyond 65536 distinct records.
+bool buildPalette(const uint8_t* mat, const double* dens, const uint8_t* meas, uint64_t n, std::vector<uint2>& palette,
+    std::vector<uint16_t>& index)
+{
+    constexpr uint32_t limit = 65536;
+    constexpr uint32_t capacity = 1u << 18; // open addressing, load <= 1/4
+    std::vector<uint64_t> keys(capacity, ~0ULL);
+    std::vector<uint32_t> slots(capacity, 0);
+    palette.clear();
+    index.resize(n);
+    uint64_t lastKey = ~0ULL;
+    uint16_t lastIndex = 0;
+    for (uint64_t i = 0; i < n; ++i) {
+        const uint32_t bits = std::bit_cast<uint32_t>(static_cast<float>(dens[i]));
+        const uint32_t tag = static_cast<uint32_t>(mat[i]) | (meas ? static_cast<uint32_t>(meas[i]) << 8 : 0u);
+        const uint64_t key = static_cast<uint64_t>(tag) << 32 | bits;
+        if (key != lastKey) {
+            uint32_t h = static_cast<uint32_t>((key * 0x9E3779B97F4A7C15ULL) >> 46);
+            while (keys[h] != key && keys[h] != ~0ULL)
+                h = (h + 1) & (capacity - 1);
+            if (keys[h] == ~0ULL) {
+                if (palette.size() == limit) {
+                    palette.clear();
+                    index.clear();
+                    return false;
+                }
+                keys[h] = key;
+                slots[h] = static_cast<uint32_t>(palette.size());
+                palette.push_back(make_uint2(bits, tag));
+            }
+            lastKey = key;
+            lastIndex = static_cast<uint16_t>(slots[h]);
+        }
+        index[i] = lastIndex;
+    }
+    return true;
 }
 
 uint32_t gridFor(uint64_t n, uint32_t block) { return static_cast<uint32_t>((n + block - 1) / block); }
@@ -364,24 +404,42 @@ int dxmc_hip_world_upload(dxmc_ctx* c, const dxmc_world_desc* d, dxmc_world** ou
     w->maxDensity.assign(d->n_materials, 0.0);
     for (uint64_t i = 0; i < n; ++i)
         w->maxDensity[d->material[i]] = std::max(w->maxDensity[d->material[i]], densSrc[i]);
+    // FP32 voxel records: palette of the distinct (density, material,
+    // measurement) records and a 1- or 2-byte index per voxel when there
+    // are at most 65536 of them (exact: the palette holds the same FP32
+    // record the voxel would)
+    std::vector<uint2> palette;
+    std::vector<uint16_t> index;
+    const bool indexed = buildPalette(d->material, densSrc, w->hasMeasurement ? d->measurement : nullptr, n, palette, index);
+    w->voxelBytes = !indexed ? 8u : palette.size() <= 256 ? 1u : 2u;
+    w->nPalette = indexed ? static_cast<uint32_t>(palette.size()) : 0u;
     int st = DXMC_OK;
-    if ((st = c->alloc(&w->material, n)) || (st = c->alloc(&w->density, n * sizeof(double))) || (st = c->alloc(&w->voxels, n * sizeof(uint2)))
-        || (w->hasMeasurement && (st = c->alloc(&w->measurement, n)))) {
+    if ((st = c->alloc(&w->material, n)) || (st = c->alloc(&w->density, n * sizeof(double))) || (st = c->alloc(&w->voxels, n * w->voxelBytes))
+        || (indexed && (st = c->alloc(&w->palette, palette.size() * sizeof(uint2)))) || (w->hasMeasurement && (st = c->alloc(&w->measurement, n)))) {
         dxmc_hip_world_destroy(c, w);
         return st;
     }
     if ((st = c->copy(w->material, d->material, n)) || (st = c->copy(w->density, densSrc, n * sizeof(double)))
-        || (w->hasMeasurement && (st = c->copy(w->measurement, d->measurement, n)))) {
+        || (w->hasMeasurement && (st = c->copy(w->measurement, d->measurement, n)))
+        || (indexed && (st = c->copy(w->palette, palette.data(), palette.size() * sizeof(uint2))))) {
         dxmc_hip_world_destroy(c, w);
         return st;
     }
-    hipLaunchKernelGGL(packVoxels, dim3(gridFor(n, 256)), dim3(256), 0, c->stream, static_cast<const uint8_t*>(w->material),
-        static_cast<const double*>(w->density), static_cast<const uint8_t*>(w->measurement), static_cast<uint2*>(w->voxels), n);
-    if (c->check(hipGetLastError()) || c->check(hipStreamSynchronize(c->stream))) {
-        dxmc_hip_world_destroy(c, w);
-        return DXMC_ERR_DEVICE;
+    if (w->voxelBytes == 2) {
+        st = c->copy(w->voxels, index.data(), n * sizeof(uint16_t));
+    } else if (w->voxelBytes == 1) {
+        std::vector<uint8_t> narrow(index.begin(), index.end());
+        st = c->copy(w->voxels, narrow.data(), n);
+    } else {
+        hipLaunchKernelGGL(packVoxels, dim3(gridFor(n, 256)), dim3(256), 0, c->stream, static_cast<const uint8_t*>(w->material),
+            static_cast<const double*>(w->density), static_cast<const uint8_t*>(w->measurement), static_cast<uint2*>(w->voxels), n);
+        st = c->check(hipGetLastError());
     }
-    w->bytes = n * (1 + sizeof(double) + sizeof(uint2) + (w->hasMeasurement ? 1 : 0));
+    if (st || c->check(hipStreamSynchronize(c->stream))) {
+        dxmc_hip_world_destroy(c, w);
+        return st ? st : DXMC_ERR_DEVICE;
+    }
+    w->bytes = n * (1 + sizeof(double) + w->voxelBytes + (w->hasMeasurement ? 1 : 0)) + w->nPalette * sizeof(uint2);
     *out = w;
     return DXMC_OK;
 }
@@ -398,6 +456,7 @@ int dxmc_hip_world_destroy(dxmc_ctx* c, dxmc_world* w)
     hipFree(w->density);
     hipFree(w->measurement);
     hipFree(w->voxels);
+    hipFree(w->palette);
     delete w;
     return DXMC_OK;
 }
@@ -525,7 +584,7 @@ int dxmc_hip_tally_normalize(dxmc_ctx* c, dxmc_tally* t, const dxmc_world* w, in
     const double vv = w ? w->spacing[0] * w->spacing[1] * w->spacing[2] / 1000.0 : 0.0;
     Tally tl { static_cast<double*>(t->dose), static_cast<uint32_t*>(t->nEvents), static_cast<double*>(t->variance) };
     hipLaunchKernelGGL(normalizeTally, dim3(gridFor(t->n, 256)), dim3(256), 0, c->stream, tl, t->n, mode, nHist, vv, calibration,
-        w ? static_cast<const double*>(w->density) : nullptr, w ? static_cast<const uint2*>(w->voxels) : nullptr);
+        w ? static_cast<const double*>(w->density) : nullptr);
     DXMC_CHECK(c, hipGetLastError());
     return DXMC_OK;
 }

@@ -6,6 +6,8 @@
 
 #include <hip/hip_runtime.h>
 #include <string>
+#include <cmath>
+#include <type_traits>
 #include <vector>
 
 struct dxmc_ctx {
@@ -69,7 +71,10 @@ struct dxmc_world {
     void* material = nullptr; // u8[n]
     void* density = nullptr; // f64[n]
     void* measurement = nullptr; // u8[n] or null
-    void* voxels = nullptr; // uint2[n] FP32 records
+    void* voxels = nullptr; // FP32 path: palette index (u8/u16) or uint2 record per voxel
+    void* palette = nullptr; // uint2[nPalette] FP32 records, null for 8-byte voxels
+    uint32_t voxelBytes = 8;
+    uint32_t nPalette = 0;
     uint64_t bytes = 0;
 };
 
@@ -153,12 +158,26 @@ inline dxmc_dev::World<T> deviceWorld(const dxmc_world* w)
     d.nz = w->dims[2];
     for (int i = 0; i < 3; ++i)
         d.spacing[i] = static_cast<T>(w->spacing[i]);
-    for (int i = 0; i < 6; ++i)
-        d.extent[i] = static_cast<T>(w->extent[i]);
+    if constexpr (std::is_same_v<T, double>) {
+        for (int i = 0; i < 6; ++i)
+            d.extent[i] = w->extent[i];
+    } else {
+        // the safe extent one ulp inside the grid, formed in T from the grid
+        // bounds as World<T>::updateMatrixExtent does (world.hpp:55-67), so
+        // that a position on a voxel face indexes the voxel below it
+        for (int i = 0; i < 3; ++i) {
+            const T lower = static_cast<T>(std::nextafter(w->extent[2 * i], -HUGE_VAL));
+            const T upper = static_cast<T>(std::nextafter(w->extent[2 * i + 1], HUGE_VAL));
+            d.extent[2 * i] = std::nextafter(lower, upper);
+            d.extent[2 * i + 1] = std::nextafter(upper, lower);
+        }
+    }
     d.material = static_cast<const uint8_t*>(w->material);
     d.density = static_cast<const double*>(w->density);
     d.measurement = static_cast<const uint8_t*>(w->measurement);
-    d.voxels = static_cast<const uint2*>(w->voxels);
+    d.voxels = w->voxels;
+    d.palette = static_cast<const uint2*>(w->palette);
+    d.voxelBytes = w->voxelBytes;
     return d;
 }
 

@@ -16,9 +16,11 @@
 // loops, fluorescence) run with many lanes active instead of with the few
 // lanes that happen to interact in a given flight, while the cheap flights
 // run nearly full. Dead lanes are refilled when at least
-// 'regeneration_threshold' of them wait (or the whole wave is dead): one lane
-// claims that many history indices with a single atomic, each dead lane takes
-// its rank among the dead lanes (ballot + mbcnt).
+// 'regeneration_threshold' of them wait (or the whole wave is dead), from a
+// wave-private pool of history indices; one lane refills the pool with a
+// single atomic on the global counter (512 histories per claim, so the
+// counter is not a point of contention); each dead lane takes its rank among
+// the dead lanes (ballot + mbcnt).
 //
 // Each history keeps its own PCG stream and consumes it in the reference's
 // order (flight, decision, interaction, roulette), so delaying an interaction
@@ -67,14 +69,18 @@ __device__ inline bool roulette(Particle<float>& p, Rng& rng)
 template <int L>
 __global__ __launch_bounds__(256) void transportPersistent(World<float> w, Tables<float> t, const Exposure<float>* exps, const uint64_t* expStart,
     uint32_t nExp, uint64_t hBegin, uint64_t hEnd, uint64_t seed, uint64_t firstHistory, Tally tally, Counters* ctr, unsigned long long* next,
-    uint32_t regenThreshold, uint32_t interactThreshold)
+    uint32_t regenThreshold, uint32_t interactThreshold, uint32_t poolSize)
 {
     Particle<float> p;
     Rng rng { 0, 1 };
     float majInv = 0;
     bool update = true;
     uint32_t phase = DEAD;
-    bool exhausted = false;
+    // the wave's pool of claimed, not yet started histories [poolNext, poolEnd)
+    const uint64_t span = hEnd - hBegin;
+    uint64_t poolNext = 0, poolEnd = 0;
+    uint32_t poolExp = 0;
+    bool claimedAll = span == 0;
     Scorer sc { tally, 0 };
     uint32_t steps = 0;
     uint32_t historySteps = 0;
@@ -89,30 +95,51 @@ __global__ __launch_bounds__(256) void transportPersistent(World<float> w, Table
     for (;;) {
         const uint64_t dead = __ballot(phase == DEAD);
         const uint32_t nDead = __popcll(dead);
-        if (!exhausted && nDead > 0 && (nDead >= regenThreshold || nDead == 64u)) {
-            const uint32_t leader = __ffsll(static_cast<unsigned long long>(dead)) - 1;
-            unsigned long long base = 0;
-            if (lane == leader)
-                base = atomicAdd(next, static_cast<unsigned long long>(nDead));
-            base = __shfl(base, static_cast<int>(leader));
-            if (hBegin + base + nDead >= hEnd)
-                exhausted = true;
-            if (phase == DEAD) {
-                const uint64_t h = hBegin + base + rankBelow(dead);
-                if (h < hEnd) {
-                    const uint32_t e = upperBound(expStart, nExp + 1, h) - 1;
+        const bool work = poolNext < poolEnd || !claimedAll;
+        if (work && nDead > 0 && (nDead >= regenThreshold || nDead == 64u)) {
+            // hand histories of the wave's pool to the dead lanes in rank
+            // order, refilling the pool from the global counter as needed
+            const uint32_t rank = rankBelow(dead);
+            uint32_t assigned = 0;
+            while (assigned < nDead) {
+                if (poolNext >= poolEnd) {
+                    if (claimedAll)
+                        break;
+                    unsigned long long base = 0;
+                    if (lane == __ffsll(static_cast<unsigned long long>(dead)) - 1)
+                        base = atomicAdd(next, static_cast<unsigned long long>(poolSize));
+                    base = __shfl(base, __ffsll(static_cast<unsigned long long>(dead)) - 1);
+                    if (base >= span) {
+                        claimedAll = true;
+                        break;
+                    }
+                    poolNext = hBegin + base;
+                    poolEnd = hBegin + (base + poolSize < span ? base + poolSize : span);
+                    if (poolEnd == hEnd)
+                        claimedAll = true;
+                    poolExp = upperBound(expStart, nExp + 1, poolNext) - 1;
+                }
+                const uint64_t avail = poolEnd - poolNext;
+                const uint32_t take = avail < nDead - assigned ? static_cast<uint32_t>(avail) : nDead - assigned;
+                if (phase == DEAD && rank >= assigned && rank < assigned + take) {
+                    const uint64_t h = poolNext + (rank - assigned);
+                    uint32_t e = poolExp;
+                    while (h >= expStart[e + 1])
+                        ++e;
                     rng = Rng::forHistory(seed, firstHistory + h);
                     sampleParticle(exps[e], t, rng, p);
                     phase = transportToWorld(w, p) ? FLYING : DEAD;
                     update = true;
                     historySteps = 0;
                 }
+                poolNext += take;
+                assigned += take;
             }
         }
         const uint64_t pend = __ballot(phase == PENDING);
         const uint64_t fly = __ballot(phase == FLYING);
         if ((pend | fly) == 0) {
-            if (exhausted)
+            if (poolNext >= poolEnd && claimedAll)
                 break;
             continue;
         }
@@ -182,18 +209,20 @@ void launchFp32(dxmc_ctx* c, const dxmc_run_cfg& cfg, const World<float>& w, con
     }
     const uint32_t regen = cfg.regeneration_threshold ? std::min<uint32_t>(cfg.regeneration_threshold, 64u) : 16u;
     const uint32_t interact = cfg.interaction_threshold ? std::min<uint32_t>(cfg.interaction_threshold, 64u) : 24u;
+    // histories claimed per atomic on the global counter
+    const uint32_t pool = 512;
     uint64_t grid = static_cast<uint64_t>(c->numCU) * perCU;
     const uint64_t lanesNeeded = (he - hb + 63) / 64 * 64;
     grid = std::max<uint64_t>(1, std::min<uint64_t>(grid, (lanesNeeded + block - 1) / block));
     const dim3 g(static_cast<uint32_t>(grid)), b(block);
     if (cfg.low_energy_correction == 0)
         hipLaunchKernelGGL(transportPersistent<0>, g, b, 0, c->stream, w, t, exps, expStart, nExp, hb, he, cfg.seed, cfg.first_history, tally, ctr, nx,
-            regen, interact);
+            regen, interact, pool);
     else if (cfg.low_energy_correction == 1)
         hipLaunchKernelGGL(transportPersistent<1>, g, b, 0, c->stream, w, t, exps, expStart, nExp, hb, he, cfg.seed, cfg.first_history, tally, ctr, nx,
-            regen, interact);
+            regen, interact, pool);
     else
         hipLaunchKernelGGL(transportPersistent<2>, g, b, 0, c->stream, w, t, exps, expStart, nExp, hb, he, cfg.seed, cfg.first_history, tally, ctr, nx,
-            regen, interact);
+            regen, interact, pool);
 }
 }
