@@ -45,6 +45,7 @@ def parse():
     p.add_argument("--threshold", type=int, default=0, help="FP32 regeneration threshold (dead lanes per wave), 0: default")
     p.add_argument("--interaction-threshold", type=int, default=0, help="FP32 pending interactions per wave before they run, 0: default")
     p.add_argument("--low-energy-correction", type=int, default=1)
+    p.add_argument("--no-autotune", action="store_true", help="skip the autotuner (defaults or the fields given)")
     return p.parse_args()
 
 
@@ -164,6 +165,16 @@ def main():
     with backend.Device(0) as dev:
         w = dev.upload_world(wl.material, wl.density, wl.spacing, ext, len(wl.names))
         t = dev.upload_tables(tables.desc())
+        tune_report = None
+        if not args.no_autotune:
+            t1 = time.perf_counter()
+            tuning, rep = dev.autotune(w, t, exposures, capi.PRECISION_FP32, L, trial_histories=1 << 23, base=tuning)
+            tuning.flags = 1
+            tune_report = {"seconds": time.perf_counter() - t1, "trials": rep.trials, "default_rate": rep.default_rate,
+                           "best_rate": rep.best_rate, "regeneration_threshold": tuning.regeneration_threshold,
+                           "interaction_threshold": tuning.interaction_threshold, "queue_depth": tuning.queue_depth,
+                           "blocks_per_cu": tuning.blocks_per_cu, "waves_per_simd": tuning.waves_per_simd}
+            print(f"autotune: {tune_report}", file=sys.stderr, flush=True)
         tally = dev.tally(n)
         for i in range(args.warmup):
             device_step(dev, w, t, tally, exposures, wl.histories, capi.PRECISION_FP32, L, tuning)
@@ -210,6 +221,7 @@ def main():
             "counters": {"steps_per_history": steps / histories, "events_per_history": events / histories,
                          "kernel_ms_per_launch": kernel_ms / max(launches, 1), "launches_per_step": launches / args.steps},
             "setup_seconds": setup_s,
+            "autotune": tune_report,
             "host_download_seconds": download_s,
             "pcie_inclusive_value": wl.histories / (sum(times) / args.steps + download_s),
         }

@@ -257,6 +257,17 @@ class Device:
                                           cancel, progress, C.byref(stats)))
         return stats
 
+    def autotune(self, world, tables, exposures, precision, low_energy_correction=1, trial_histories=1 << 23, base=None):
+        """dxmc_hip_autotune: tuned run configuration and its report."""
+        cfg = base if base is not None else capi.RunCfg()
+        cfg.precision = precision
+        cfg.low_energy_correction = low_energy_correction
+        best = capi.RunCfg()
+        rep = capi.TuneReport()
+        self._check(self.lib.dxmc_hip_autotune(self.ctx, world, tables, exposures, len(exposures), C.byref(cfg), trial_histories, C.byref(best),
+                                               C.byref(rep)))
+        return best, rep
+
     def synchronize(self):
         self._check(self.lib.dxmc_hip_synchronize(self.ctx))
 

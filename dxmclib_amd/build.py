@@ -35,6 +35,7 @@ HIP_SOURCES = {
     "dxmc_hip.hip": ["-ffp-contract=off"],
     "tables.hip": ["-ffp-contract=off"],
     "transport_fp32.hip": [],
+    "autotune.hip": [],
 }
 HOST_SOURCES = ["src/material.cpp", "src/host_capi.cpp"]
 HOST_FLAGS = ["-O2", "-std=c++20", "-fPIC", "-ffp-contract=off",
@@ -84,8 +85,26 @@ def build_host(verbose=False):
     return HOST_LIB
 
 
+# programs written against the C++ host API the way a user of the reference
+# API writes them (tests/cpp), linked to the two libraries
+PROGRAMS = {"layered_pencil": os.path.join(ROOT, "tests", "cpp", "layered_pencil.cpp")}
+PROGRAM_DIR = os.path.join(ROOT, "tests", "cpp", "bin")
+
+
+def build_programs(verbose=False):
+    os.makedirs(PROGRAM_DIR, exist_ok=True)
+    out = []
+    for name, src in PROGRAMS.items():
+        exe = os.path.join(PROGRAM_DIR, name)
+        _run(["g++", "-O2", "-std=c++20", "-I", os.path.join(HERE, "include"), "-I", os.path.join(ROOT, "include"), src,
+              "-o", exe + ".tmp", "-L", HERE, "-ldxmc_host", "-ldxmc_hip", "-Wl,-rpath,$ORIGIN/../../../dxmclib_amd", "-lpthread"])
+        os.replace(exe + ".tmp", exe)
+        out.append(exe)
+    return out
+
+
 def build_all(verbose=False):
-    return build_hip(verbose), build_host(verbose)
+    return build_hip(verbose), build_host(verbose), build_programs(verbose)
 
 
 if __name__ == "__main__":

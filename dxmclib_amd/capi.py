@@ -25,6 +25,8 @@ PRECISION_FP64 = 0
 PRECISION_FP32 = 1
 OUTPUT_EV_PER_HISTORY = 0
 OUTPUT_DOSE = 1
+RUN_COUNTERS = 1
+RUN_AUTOTUNE = 2
 
 SOURCE_PENCIL = 0
 SOURCE_ISOTROPIC = 1
@@ -134,7 +136,18 @@ class RunCfg(C.Structure):
         ("regeneration_threshold", C.c_uint32),
         ("flags", C.c_uint32),
         ("interaction_threshold", C.c_uint32),
+        ("queue_depth", C.c_uint32),
+        ("waves_per_simd", C.c_uint32),
         ("reserved", C.c_uint32),
+    ]
+
+
+class TuneReport(C.Structure):
+    _fields_ = [
+        ("trials", C.c_uint32),
+        ("reserved", C.c_uint32),
+        ("default_rate", C.c_double),
+        ("best_rate", C.c_double),
     ]
 
 
@@ -236,6 +249,8 @@ HIP_SYMBOLS = {
     "dxmc_hip_tally_normalize": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int, C.c_uint64, C.c_double]),
     "dxmc_hip_run": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.POINTER(Exposure), C.c_uint64, C.POINTER(RunCfg), C.c_void_p,
                                C.POINTER(C.c_int32), C.POINTER(C.c_uint64), C.POINTER(RunStats)]),
+    "dxmc_hip_autotune": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.POINTER(Exposure), C.c_uint64, C.POINTER(RunCfg), C.c_uint64,
+                                    C.POINTER(RunCfg), C.POINTER(TuneReport)]),
     "dxmc_hip_rng_streams": (C.c_int, [C.c_void_p, C.c_uint64, C.c_uint64, C.c_uint64, C.c_uint32, u32p]),
     "dxmc_hip_lut_lookup": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int32, C.c_uint64, u32p, f64p, f64p, f64p]),
     "dxmc_hip_interact": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int32, C.c_int32, C.c_int32, C.c_uint64, u32p, C.c_double, C.c_uint64,

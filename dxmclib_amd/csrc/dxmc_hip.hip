@@ -450,6 +450,7 @@ int dxmc_hip_world_destroy(dxmc_ctx* c, dxmc_world* w)
         return DXMC_ERR_ARGUMENT;
     if (w->ctx != c)
         return c->fail(DXMC_ERR_STATE, "world_destroy: world belongs to another context");
+    std::erase_if(c->tuned, [w](const auto& kv) { return std::get<0>(kv.first) == w; });
     hipSetDevice(c->device);
     hipStreamSynchronize(c->stream);
     hipFree(w->material);
@@ -497,6 +498,7 @@ int dxmc_hip_tables_destroy(dxmc_ctx* c, dxmc_tables* t)
         return DXMC_ERR_ARGUMENT;
     if (t->ctx != c)
         return c->fail(DXMC_ERR_STATE, "tables_destroy: tables belong to another context");
+    std::erase_if(c->tuned, [t](const auto& kv) { return std::get<1>(kv.first) == t; });
     hipSetDevice(c->device);
     hipStreamSynchronize(c->stream);
     for (void* p : t->allocations)
@@ -594,12 +596,49 @@ int dxmc_hip_tally_normalize(dxmc_ctx* c, dxmc_tally* t, const dxmc_world* w, in
 int dxmc_hip_run(dxmc_ctx* c, const dxmc_world* w, const dxmc_tables* t, const dxmc_exposure* exposures, uint64_t nExp, const dxmc_run_cfg* cfg,
     dxmc_tally* tally, const volatile int32_t* cancel, volatile uint64_t* exposuresDone, dxmc_run_stats* stats)
 {
-    if (!c || !w || !t || !exposures || !cfg || !tally)
+    if (!tally)
         return c ? c->fail(DXMC_ERR_ARGUMENT, "run: NULL argument") : DXMC_ERR_ARGUMENT;
-    if (w->ctx != c || t->ctx != c || tally->ctx != c)
+    int st = validateRun(c, w, t, exposures, nExp, cfg, tally);
+    if (st)
+        return st;
+    DXMC_CHECK(c, hipSetDevice(c->device));
+    dxmc_run_cfg run = *cfg;
+    if (run.flags & DXMC_RUN_AUTOTUNE) {
+        const auto key = tuneKey(w, t, run);
+        auto it = c->tuned.find(key);
+        if (it == c->tuned.end()) {
+            uint64_t total = 0;
+            for (uint64_t i = 0; i < nExp; ++i)
+                total += exposures[i].n_histories;
+            dxmc_run_cfg best {};
+            const uint64_t trial = std::clamp<uint64_t>(total / 8, uint64_t(1) << 20, uint64_t(1) << 23);
+            if ((st = autotune(c, w, t, exposures, nExp, run, trial, best, nullptr)))
+                return st;
+            it = c->tuned.emplace(key, best).first;
+        }
+        const dxmc_run_cfg& b = it->second;
+        for (auto f : { &dxmc_run_cfg::block_size, &dxmc_run_cfg::blocks_per_cu, &dxmc_run_cfg::regeneration_threshold,
+                 &dxmc_run_cfg::interaction_threshold, &dxmc_run_cfg::queue_depth, &dxmc_run_cfg::waves_per_simd })
+            if (run.*f == 0)
+                run.*f = b.*f;
+    }
+    return runTransport(c, w, t, exposures, nExp, run, tally, cancel, exposuresDone, stats);
+}
+
+} // extern "C"
+
+namespace dxmc_backend {
+int validateRun(dxmc_ctx* c, const dxmc_world* w, const dxmc_tables* t, const dxmc_exposure* exposures, uint64_t nExp, const dxmc_run_cfg* cfg,
+    const dxmc_tally* tally)
+{
+    if (!c || !w || !t || !exposures || !cfg)
+        return c ? c->fail(DXMC_ERR_ARGUMENT, "run: NULL argument") : DXMC_ERR_ARGUMENT;
+    if (w->ctx != c || t->ctx != c || (tally && tally->ctx != c))
         return c->fail(DXMC_ERR_STATE, "run: objects belong to another context");
-    if (tally->n != w->n)
+    if (tally && tally->n != w->n)
         return c->fail(DXMC_ERR_ARGUMENT, "run: tally size differs from the world size");
+    if (cfg->precision == DXMC_PRECISION_FP64 && cfg->block_size > 256)
+        return c->fail(DXMC_ERR_ARGUMENT, "run: FP64 block size must be at most 256");
     if (t->nMaterials < w->nMaterials)
         return c->fail(DXMC_ERR_ARGUMENT, "run: tables hold fewer materials than the world uses");
     if (cfg->low_energy_correction < 0 || cfg->low_energy_correction > 2 || (cfg->precision != DXMC_PRECISION_FP64 && cfg->precision != DXMC_PRECISION_FP32))
@@ -611,8 +650,11 @@ int dxmc_hip_run(dxmc_ctx* c, const dxmc_world* w, const dxmc_tables* t, const d
         if (e.spectrum >= static_cast<int32_t>(t->nSpectra) || e.heel >= static_cast<int32_t>(t->nHeel) || e.bowtie >= static_cast<int32_t>(t->nBowtie))
             return c->fail(DXMC_ERR_ARGUMENT, "run: exposure refers to a missing spectrum, heel or bowtie table");
     }
-    return runTransport(c, w, t, exposures, nExp, *cfg, tally, cancel, exposuresDone, stats);
+    return DXMC_OK;
 }
+}
+
+extern "C" {
 
 // ----------------------------------------------------------------- parity
 
@@ -755,10 +797,10 @@ int runTransport(dxmc_ctx* c, const dxmc_world* w, const dxmc_tables* t, const d
     const uint64_t total = rb.totalHistories;
     uint64_t chunk = cfg.histories_per_launch;
     if (chunk == 0)
-        chunk = cfg.precision == DXMC_PRECISION_FP64 ? (uint64_t(1) << 22) : (uint64_t(1) << 25);
+        chunk = cfg.precision == DXMC_PRECISION_FP64 ? (uint64_t(1) << 22) : (uint64_t(1) << 27);
     const uint32_t block = cfg.block_size ? cfg.block_size : 256;
-    if (block % 64 != 0 || block > 1024)
-        return c->fail(DXMC_ERR_ARGUMENT, "run: block size must be a multiple of 64 up to 1024");
+    if (block % 64 != 0 || block > 256)
+        return c->fail(DXMC_ERR_ARGUMENT, "run: block size must be a multiple of 64 up to 256");
     double kernelMs = 0;
     uint64_t launches = 0;
     uint64_t done = 0;

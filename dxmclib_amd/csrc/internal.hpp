@@ -5,7 +5,9 @@
 #include "dxmc_hip.h"
 
 #include <hip/hip_runtime.h>
+#include <map>
 #include <string>
+#include <tuple>
 #include <cmath>
 #include <type_traits>
 #include <vector>
@@ -18,6 +20,8 @@ struct dxmc_ctx {
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     void* historyCounter = nullptr; // u64 work counter of the persistent kernel
     std::string error;
+    // autotuner results per (world, tables, precision, low energy correction)
+    std::map<std::tuple<const void*, const void*, int, int>, dxmc_run_cfg> tuned;
 
     int fail(int status, const std::string& msg)
     {
@@ -206,6 +210,18 @@ __device__ inline void accumulateCounters(dxmc_dev::Counters* ctr, uint32_t step
 bool validateTables(const dxmc_tables_desc& d, std::string& why);
 int buildDeviceTables(dxmc_ctx* c, const dxmc_tables_desc& d, dxmc_tables& t);
 int uploadExposures(dxmc_ctx* c, const dxmc_exposure* x, uint64_t nExp, RunBuffers& rb);
+// argument checks shared by dxmc_hip_run and dxmc_hip_autotune (tally may be null)
+int validateRun(dxmc_ctx* c, const dxmc_world* w, const dxmc_tables* t, const dxmc_exposure* x, uint64_t nExp, const dxmc_run_cfg* cfg,
+    const dxmc_tally* tally);
+inline std::tuple<const void*, const void*, int, int> tuneKey(const dxmc_world* w, const dxmc_tables* t, const dxmc_run_cfg& cfg)
+{
+    return { w, t, cfg.precision, cfg.low_energy_correction };
+}
+// autotune.hip
+int autotune(dxmc_ctx* c, const dxmc_world* w, const dxmc_tables* t, const dxmc_exposure* x, uint64_t nExp, const dxmc_run_cfg& base,
+    uint64_t trialHistories, dxmc_run_cfg& best, dxmc_tune_report* report);
+// resident blocks per CU of the FP32 kernel (transport_fp32.hip)
+uint32_t residentBlocksFp32(int L, uint32_t wavesPerSimd);
 int runTransport(dxmc_ctx* c, const dxmc_world* w, const dxmc_tables* t, const dxmc_exposure* x, uint64_t nExp, const dxmc_run_cfg& cfg,
     dxmc_tally* tally, const volatile int32_t* cancel, volatile uint64_t* exposuresDone, dxmc_run_stats* stats);
 // FP32 transport launch (transport_fp32.hip)

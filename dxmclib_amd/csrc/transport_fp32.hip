@@ -55,6 +55,18 @@ __device__ inline uint64_t voxelIndexFast(const World<float>& w, const float* po
     return (static_cast<uint64_t>(iz) * w.ny + iy) * w.nx + ix;
 }
 
+// Position of an energy in the FP32 LUT: segment k and the offset of
+// log2 E from the segment's anchor node. It changes only with the photon's
+// energy, so the kernel keeps it per photon and each flight evaluates
+// mu = 2^(y_k + a_k * dl) for the voxel's material without the log and the
+// segment search (same arithmetic as attenuation() / majorantInverse()).
+__device__ inline void lutPoint(const Tables<float>& t, float E, uint32_t& k, float& dl)
+{
+    const float le = Lut<float>::logE(E);
+    k = lutInterval(t, le);
+    dl = le - t.lutGrid[k];
+}
+
 // Russian roulette after a step that left the photon alive (transport.hpp:684-694)
 __device__ inline bool roulette(Particle<float>& p, Rng& rng)
 {
@@ -66,8 +78,11 @@ __device__ inline bool roulette(Particle<float>& p, Rng& rng)
     return true;
 }
 
-template <int L>
-__global__ __launch_bounds__(256) void transportPersistent(World<float> w, Tables<float> t, const Exposure<float>* exps, const uint64_t* expStart,
+// W: minimum resident waves per SIMD the register allocation must allow
+// (1: unconstrained, about 5 with this kernel; 6 and 8 trade a few spills
+// for occupancy). The autotuner picks among the variants.
+template <int L, int W>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(W, 8))) void transportPersistent(World<float> w, Tables<float> t, const Exposure<float>* exps, const uint64_t* expStart,
     uint32_t nExp, uint64_t hBegin, uint64_t hEnd, uint64_t seed, uint64_t firstHistory, Tally tally, Counters* ctr, unsigned long long* next,
     uint32_t regenThreshold, uint32_t interactThreshold, uint32_t poolSize)
 {
@@ -87,8 +102,9 @@ __global__ __launch_bounds__(256) void transportPersistent(World<float> w, Table
     // the pending interaction
     uint64_t idx = 0;
     uint32_t mat = 0, meas = 0;
-    float att[3] = { 0, 0, 0 };
-    float prob = 0;
+    float density = 0;
+    uint32_t lutK = 0;
+    float lutDl = 0;
     const float inv[3] = { 1.0f / w.spacing[0], 1.0f / w.spacing[1], 1.0f / w.spacing[2] };
     const uint32_t lane = __lane_id();
 
@@ -146,6 +162,12 @@ __global__ __launch_bounds__(256) void transportPersistent(World<float> w, Table
         const uint32_t nPend = __popcll(pend);
         if (nPend != 0 && (nPend >= interactThreshold || fly == 0)) {
             if (phase == PENDING) {
+                float att[3];
+                const float* cf = t.lutCoef + (static_cast<size_t>(mat) * t.lutNodes + lutK) * 6;
+                att[0] = exp2f(cf[0] + cf[1] * lutDl);
+                att[1] = exp2f(cf[2] + cf[3] * lutDl);
+                att[2] = exp2f(cf[4] + cf[5] * lutDl);
+                const float prob = ((att[0] + att[1]) + att[2]) * density * majInv;
                 bool alive = meas == 0 ? computeInteractions<float, L>(t, att, p, mat, sc, idx, rng, update)
                                        : computeInteractionsForced<float, L>(t, prob, att, p, mat, sc, idx, rng, update);
                 if (alive)
@@ -154,7 +176,9 @@ __global__ __launch_bounds__(256) void transportPersistent(World<float> w, Table
             }
         } else if (phase == FLYING) {
             if (update) {
-                majInv = majorantInverse(t, p.energy);
+                lutPoint(t, p.energy, lutK, lutDl);
+                const float* m = t.lutMajorant + lutK * 2;
+                majInv = exp2f(m[0] + m[1] * lutDl);
                 update = false;
             }
             const float step = -logf(rng.uniform<float>()) * majInv * 10.0f; // cm -> mm
@@ -165,10 +189,13 @@ __global__ __launch_bounds__(256) void transportPersistent(World<float> w, Table
             } else {
                 ++steps;
                 idx = voxelIndexFast(w, p.pos, inv);
-                float density;
                 readVoxel(w, idx, mat, density, meas);
-                attenuation(t, mat, p.energy, att);
-                prob = ((att[0] + att[1]) + att[2]) * density * majInv;
+                float att[3];
+                const float* cf = t.lutCoef + (static_cast<size_t>(mat) * t.lutNodes + lutK) * 6;
+                att[0] = exp2f(cf[0] + cf[1] * lutDl);
+                att[1] = exp2f(cf[2] + cf[3] * lutDl);
+                att[2] = exp2f(cf[4] + cf[5] * lutDl);
+                const float prob = ((att[0] + att[1]) + att[2]) * density * majInv;
                 if (meas == 0) {
                     if (rng.uniform<float>() < prob)
                         phase = PENDING;
@@ -190,39 +217,56 @@ __global__ __launch_bounds__(256) void transportPersistent(World<float> w, Table
 
 namespace dxmc_backend {
 
+namespace {
+
+using KernelFn = void (*)(World<float>, Tables<float>, const Exposure<float>*, const uint64_t*, uint32_t, uint64_t, uint64_t, uint64_t, uint64_t,
+    Tally, Counters*, unsigned long long*, uint32_t, uint32_t, uint32_t);
+
+template <int L>
+KernelFn variant(uint32_t waves)
+{
+    if (waves >= 8)
+        return transportPersistent<L, 8>;
+    if (waves >= 6)
+        return transportPersistent<L, 6>;
+    return transportPersistent<L, 1>;
+}
+
+KernelFn kernelFor(int L, uint32_t waves)
+{
+    return L == 0 ? variant<0>(waves) : L == 1 ? variant<1>(waves) : variant<2>(waves);
+}
+
+}
+
+uint32_t residentBlocksFp32(int L, uint32_t waves)
+{
+    int resident = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&resident, reinterpret_cast<const void*>(kernelFor(L, waves)), 256, 0) != hipSuccess
+        || resident <= 0) {
+        hipGetLastError();
+        resident = 4;
+    }
+    return static_cast<uint32_t>(resident);
+}
+
 void launchFp32(dxmc_ctx* c, const dxmc_run_cfg& cfg, const World<float>& w, const Tables<float>& t, const Exposure<float>* exps,
     const uint64_t* expStart, uint32_t nExp, uint64_t hb, uint64_t he, const Tally& tally, Counters* ctr)
 {
     auto* nx = static_cast<unsigned long long*>(c->historyCounter);
     hipMemsetAsync(nx, 0, sizeof(unsigned long long), c->stream);
     const uint32_t block = 256;
+    const uint32_t waves = cfg.waves_per_simd;
     // default: as many blocks as stay resident on a CU (register-limited)
-    uint32_t perCU = cfg.blocks_per_cu;
-    if (perCU == 0) {
-        int resident = 0;
-        const void* k = cfg.low_energy_correction == 0 ? reinterpret_cast<const void*>(&transportPersistent<0>)
-            : cfg.low_energy_correction == 1           ? reinterpret_cast<const void*>(&transportPersistent<1>)
-                                                       : reinterpret_cast<const void*>(&transportPersistent<2>);
-        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&resident, k, static_cast<int>(block), 0) != hipSuccess || resident <= 0)
-            resident = 4;
-        perCU = static_cast<uint32_t>(resident);
-    }
+    const uint32_t perCU = cfg.blocks_per_cu ? cfg.blocks_per_cu : residentBlocksFp32(cfg.low_energy_correction, waves);
     const uint32_t regen = cfg.regeneration_threshold ? std::min<uint32_t>(cfg.regeneration_threshold, 64u) : 16u;
     const uint32_t interact = cfg.interaction_threshold ? std::min<uint32_t>(cfg.interaction_threshold, 64u) : 24u;
-    // histories claimed per atomic on the global counter
-    const uint32_t pool = 512;
+    // histories a wave claims per atomic on the global counter
+    const uint32_t pool = cfg.queue_depth ? std::clamp<uint32_t>(cfg.queue_depth, 64u, 1u << 16) : 512u;
     uint64_t grid = static_cast<uint64_t>(c->numCU) * perCU;
     const uint64_t lanesNeeded = (he - hb + 63) / 64 * 64;
     grid = std::max<uint64_t>(1, std::min<uint64_t>(grid, (lanesNeeded + block - 1) / block));
-    const dim3 g(static_cast<uint32_t>(grid)), b(block);
-    if (cfg.low_energy_correction == 0)
-        hipLaunchKernelGGL(transportPersistent<0>, g, b, 0, c->stream, w, t, exps, expStart, nExp, hb, he, cfg.seed, cfg.first_history, tally, ctr, nx,
-            regen, interact, pool);
-    else if (cfg.low_energy_correction == 1)
-        hipLaunchKernelGGL(transportPersistent<1>, g, b, 0, c->stream, w, t, exps, expStart, nExp, hb, he, cfg.seed, cfg.first_history, tally, ctr, nx,
-            regen, interact, pool);
-    else
-        hipLaunchKernelGGL(transportPersistent<2>, g, b, 0, c->stream, w, t, exps, expStart, nExp, hb, he, cfg.seed, cfg.first_history, tally, ctr, nx,
-            regen, interact, pool);
+    hipLaunchKernelGGL(kernelFor(cfg.low_energy_correction, waves), dim3(static_cast<uint32_t>(grid)), dim3(block), 0, c->stream, w, t, exps,
+        expStart, nExp, hb, he, cfg.seed, cfg.first_history, tally, ctr, nx, regen, interact, pool);
 }
 }

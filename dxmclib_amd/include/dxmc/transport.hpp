@@ -69,7 +69,8 @@ class Source;
 // One HIP context per device, shared by all Transport objects of the process.
 class DeviceContexts {
 public:
-    static dxmc_ctx* get(int device)
+    // the device's context, or nullptr with the reason in 'why'
+    static dxmc_ctx* get(int device, std::string& why)
     {
         static std::mutex mutex;
         static std::map<int, std::unique_ptr<dxmc_ctx, int (*)(dxmc_ctx*)>> contexts;
@@ -78,8 +79,10 @@ public:
         if (it != contexts.end())
             return it->second.get();
         dxmc_ctx* c = nullptr;
-        if (dxmc_hip_ctx_create(device, &c) != DXMC_OK)
-            throw std::runtime_error(std::string("dxmc: cannot create a HIP context: ") + dxmc_hip_last_error(nullptr));
+        if (dxmc_hip_ctx_create(device, &c) != DXMC_OK) {
+            why = std::string("cannot create a HIP context: ") + dxmc_hip_last_error(nullptr);
+            return nullptr;
+        }
         contexts.emplace(device, std::unique_ptr<dxmc_ctx, int (*)(dxmc_ctx*)>(c, dxmc_hip_ctx_destroy));
         return c;
     }
@@ -187,7 +190,9 @@ protected:
         ProgressBar<T>* pb, Calibration calibration, std::vector<double>& dose, std::vector<std::uint32_t>& nEvents, std::vector<double>& variance,
         std::uint64_t nHist)
     {
-        dxmc_ctx* c = DeviceContexts::get(m_device);
+        dxmc_ctx* c = DeviceContexts::get(m_device, m_error);
+        if (!c)
+            return false;
         dxmc_world_desc wd {};
         for (int i = 0; i < 3; ++i) {
             wd.dims[i] = world.dimensions()[i];

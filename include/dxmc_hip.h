@@ -187,10 +187,16 @@ typedef struct {
     uint32_t block_size; /* 0: tuned default */
     uint32_t blocks_per_cu; /* FP32 persistent kernel occupancy, 0: tuned default */
     uint32_t regeneration_threshold; /* FP32: dead lanes per wave before refill, 0: tuned */
-    uint32_t flags; /* bit 0: collect step/event counters */
+    uint32_t flags; /* DXMC_RUN_* bits */
     uint32_t interaction_threshold; /* FP32: pending interactions per wave before they run, 0: tuned */
-    uint32_t reserved; /* must be 0 */
+    uint32_t queue_depth; /* FP32: histories a wave claims per refill of its pool, 0: tuned */
+    uint32_t waves_per_simd; /* FP32: kernel variant by register budget, 5 (0), 6 or 8 resident waves per SIMD */
+    uint32_t reserved; /* 0 */
 } dxmc_run_cfg;
+
+/* dxmc_run_cfg.flags */
+#define DXMC_RUN_COUNTERS 1u /* collect step/event counters */
+#define DXMC_RUN_AUTOTUNE 2u /* fill zero tuning fields from the autotuner (tuned once per world/tables/precision/L and kept in the context) */
 
 typedef struct {
     uint64_t histories; /* histories started */
@@ -208,6 +214,27 @@ typedef struct {
 int dxmc_hip_run(dxmc_ctx* ctx, const dxmc_world* world, const dxmc_tables* tables, const dxmc_exposure* exposures,
     uint64_t n_exposures, const dxmc_run_cfg* cfg, dxmc_tally* tally, const volatile int32_t* cancel,
     volatile uint64_t* exposures_done, dxmc_run_stats* stats);
+
+/* Autotuner of the transport kernels (no counterpart in the reference, whose
+ * CPU threads need no tuning). Times trial launches of trial_histories
+ * histories drawn evenly from the exposure table into a scratch tally and
+ * searches, one parameter at a time from the defaults, over
+ *   FP32: waves_per_simd, regeneration_threshold, interaction_threshold,
+ *         queue_depth, blocks_per_cu;
+ *   FP64: block_size.
+ * Tuning fields that are non-zero in 'base' are kept fixed. 'best' receives
+ * 'base' with the chosen values filled in; the choice is also kept in the
+ * context for DXMC_RUN_AUTOTUNE runs on the same world and tables. The
+ * tuning fields change speed only: per-history results do not depend on
+ * them. */
+typedef struct {
+    uint32_t trials; /* trial launches timed */
+    uint32_t reserved;
+    double default_rate; /* histories/s with all tuning fields 0 */
+    double best_rate; /* histories/s with 'best' */
+} dxmc_tune_report;
+int dxmc_hip_autotune(dxmc_ctx* ctx, const dxmc_world* world, const dxmc_tables* tables, const dxmc_exposure* exposures, uint64_t n_exposures,
+    const dxmc_run_cfg* base, uint64_t trial_histories, dxmc_run_cfg* best, dxmc_tune_report* report);
 
 /* ------------------------------------------------------ parity entry points */
 /* PCG32 streams (RandomState::pcg32, dxmcrandom.hpp:156-165) of histories

@@ -36,7 +36,7 @@ def test_struct_layouts():
     assert C.sizeof(capi.Exposure) == 18 * 8 + 8 + 16
     assert C.sizeof(capi.Shell) == 11 * 8
     assert C.sizeof(capi.Heel) == 4 * 8 + 16
-    assert C.sizeof(capi.RunCfg) == 56
+    assert C.sizeof(capi.RunCfg) == 64
 
 
 def test_null_arguments_rejected():

@@ -264,3 +264,43 @@ def test_fp32_voxel_records(device, records):
     assert frac >= 0.997
     assert abs(s32.steps - s64.steps) <= 1e-3 * s64.steps
     assert abs(s32.events - s64.events) <= 1e-3 * s64.events
+
+
+def test_autotune_keeps_results(device):
+    """The autotuner changes scheduling only: a run with the tuned
+    configuration follows the same histories as one with the defaults."""
+    wl = workloads.dx_thorax(histories_per_exposure=4000, exposures=50)
+    ext = backend.world_extent(wl.material.shape[::-1], wl.spacing)
+    t = backend.HostTables(wl.names, [0.001205, 1.0, 0.95, 1.92, 0.26], 120.0, 1.0)
+    exps = t.exposures(wl.spec)
+    w = device.upload_world(wl.material, wl.density, wl.spacing, ext, len(wl.names))
+    tt = device.upload_tables(t.desc())
+    try:
+        best, rep = device.autotune(w, tt, exps, capi.PRECISION_FP32, 1, trial_histories=1 << 20)
+        print(f"autotune: {rep.trials} trials, default {rep.default_rate:.3e} best {rep.best_rate:.3e} hist/s; "
+              f"regen {best.regeneration_threshold} interact {best.interaction_threshold} queue {best.queue_depth} "
+              f"blocks/CU {best.blocks_per_cu} waves/SIMD {best.waves_per_simd}", flush=True)
+        assert rep.trials >= 10
+        assert rep.best_rate >= rep.default_rate * 0.9
+        out = []
+        for cfg in (capi.RunCfg(), best):
+            cfg.flags = capi.RUN_COUNTERS
+            tally = device.tally(wl.voxels)
+            stats = device.run(w, tt, exps, tally, capi.PRECISION_FP32, 1, seed=42, tuning=cfg)
+            d, ev, v = device.download(tally, wl.voxels)
+            device.destroy_tally(tally)
+            out.append((stats.steps, stats.events, d, ev))
+        assert out[0][0] == out[1][0] and out[0][1] == out[1][1]
+        assert np.array_equal(out[0][3], out[1][3])
+        # FP64 atomics add in a different order: rounding only
+        assert np.allclose(out[0][2], out[1][2], rtol=1e-10, atol=0)
+        # DXMC_RUN_AUTOTUNE uses the tuned values kept in the context
+        cfg = capi.RunCfg()
+        cfg.flags = capi.RUN_COUNTERS | capi.RUN_AUTOTUNE
+        tally = device.tally(wl.voxels)
+        stats = device.run(w, tt, exps, tally, capi.PRECISION_FP32, 1, seed=42, tuning=cfg)
+        device.destroy_tally(tally)
+        assert stats.steps == out[0][0] and stats.events == out[0][1]
+    finally:
+        device.destroy_tables(tt)
+        device.destroy_world(w)
