@@ -44,6 +44,8 @@ def parse():
     p.add_argument("--blocks-per-cu", type=int, default=0)
     p.add_argument("--threshold", type=int, default=0, help="FP32 regeneration threshold (dead lanes per wave), 0: default")
     p.add_argument("--interaction-threshold", type=int, default=0, help="FP32 pending interactions per wave before they run, 0: default")
+    p.add_argument("--queue-depth", type=int, default=0, help="FP32 histories per pool refill, 0: default")
+    p.add_argument("--waves-per-simd", type=int, default=0, help="FP32 kernel variant (5, 6, 8), 0: default")
     p.add_argument("--low-energy-correction", type=int, default=1)
     p.add_argument("--no-autotune", action="store_true", help="skip the autotuner (defaults or the fields given)")
     return p.parse_args()
@@ -160,6 +162,8 @@ def main():
     tuning.blocks_per_cu = args.blocks_per_cu
     tuning.regeneration_threshold = args.threshold
     tuning.interaction_threshold = args.interaction_threshold
+    tuning.queue_depth = args.queue_depth
+    tuning.waves_per_simd = args.waves_per_simd
     tuning.flags = 1
 
     with backend.Device(0) as dev:

@@ -80,9 +80,11 @@ struct Tables {
     uint32_t lutNodes;
     uint32_t lutLinearIndex;
     uint32_t nSpectra;
-    T lutLinearStep;
-    T lutLinearEnergy;
-    const T* lutGrid;
+    double lutLinearStep; // log10 units, both precisions
+    double lutLinearEnergy;
+    const double* lutGrid; // log10 E of the nodes, both precisions
+    const double* lutBound; // FP32: lowest energy of each segment on the FP64 path
+    const float* lutAnchor; // FP32: log2 E of the nodes
     const T* lutCoef; // [mat][node][type][b,a]
     const T* lutMajorant; // [node][b,a]
     const T* rita; // [mat][x,e,a,b][56]
@@ -217,30 +219,17 @@ __device__ inline void peturb(T* dir, T theta, T phi)
 
 // ------------------------------------------------------ attenuation tables
 
+// Segment search in log10 E on the FP64 grid for both precisions
+// (attenuationinterpolator.hpp:207-229, linear index with its lag above the
+// last edge, binary search below): the FP32 path then uses the same segment
+// as the FP64 path, also for energies that sit on a grid node, where float
+// rounding of log E would otherwise pick the neighbouring segment.
 template <typename T>
-struct Lut;
-
-// FP64: mu = 10^(b + a log10 E) exactly as the reference.
-// FP32: the host stores each segment anchored at its upper grid node x_k
-// (log2 units): mu = 2^(y_k + a (log2 E - x_k)), y_k = log2 mu(x_k). This
-// avoids the cancellation of b + a log E in float for steep segments.
-template <>
-struct Lut<double> {
-    __device__ static inline double logE(double E) { return log10(E); }
-    __device__ static inline double value(double b, double a, double le, double) { return pow(10.0, b + a * le); }
-};
-template <>
-struct Lut<float> {
-    __device__ static inline float logE(float E) { return log2f(E); }
-    __device__ static inline float value(float y, float a, float le, float anchor) { return exp2f(y + a * (le - anchor)); }
-};
-
-template <typename T>
-__device__ inline uint32_t lutInterval(const Tables<T>& t, T le)
+__device__ inline uint32_t lutInterval(const Tables<T>& t, double le)
 {
     if (le > t.lutLinearEnergy) {
-        const T f = (le - t.lutLinearEnergy) / t.lutLinearStep;
-        if (!(f < T(t.lutNodes))) // beyond the table (or NaN): last interval
+        const double f = (le - t.lutLinearEnergy) / t.lutLinearStep;
+        if (!(f < double(t.lutNodes))) // beyond the table (or NaN): last interval
             return t.lutNodes - 1;
         const uint32_t k = static_cast<uint32_t>(f) + t.lutLinearIndex;
         return k < t.lutNodes - 1 ? k : t.lutNodes - 1;
@@ -249,26 +238,70 @@ __device__ inline uint32_t lutInterval(const Tables<T>& t, T le)
     return up != t.lutNodes ? up : t.lutNodes - 1;
 }
 
-// photo, Compton, Rayleigh mass attenuation [cm2/g]
+// An energy's place in the LUT: segment k and
+//   FP64: x = log10 E, mu = 10^(b + a x) exactly as the reference;
+//   FP32: x = log2 E - X_k, the offset from the segment's anchor node X_k
+//         (log2 units), mu = 2^(y_k + a x) with y_k = log2 mu(X_k) stored by
+//         the host. This avoids the cancellation of b + a log E in float for
+//         steep segments.
+template <typename T>
+struct LutPoint {
+    uint32_t k;
+    T x;
+};
+
+__device__ inline LutPoint<double> lutPoint(const Tables<double>& t, double E)
+{
+    const double le = log10(E);
+    return { lutInterval(t, le), le };
+}
+// FP32: a first segment from log2f, then corrected against the segment
+// bounds (lutBound[k] = smallest FP64 energy the FP64 path places in segment
+// k or above, found on the device with the FP64 code when the tables are
+// uploaded): energies on or next to a node get the FP64 path's segment
+// without an FP64 logarithm.
+__device__ inline LutPoint<float> lutPoint(const Tables<float>& t, float E)
+{
+    constexpr float LOG10_2 = 0.30102999566398119521f;
+    const float le2 = log2f(E);
+    uint32_t k = lutInterval(t, static_cast<double>(le2 * LOG10_2));
+    const double e = E;
+    while (k + 1 < t.lutNodes && e >= t.lutBound[k + 1])
+        ++k;
+    while (k > 0 && e < t.lutBound[k])
+        --k;
+    return { k, le2 - t.lutAnchor[k] };
+}
+__device__ inline double lutValue(const double* c, const LutPoint<double>& p) { return pow(10.0, c[0] + c[1] * p.x); }
+__device__ inline float lutValue(const float* c, const LutPoint<float>& p) { return exp2f(c[0] + c[1] * p.x); }
+
+// photo, Compton, Rayleigh mass attenuation [cm2/g] of a material at a LUT point
+template <typename T>
+__device__ inline void attenuationAt(const Tables<T>& t, uint32_t mat, const LutPoint<T>& p, T att[3])
+{
+    const T* c = t.lutCoef + (static_cast<size_t>(mat) * t.lutNodes + p.k) * 6;
+    att[0] = lutValue(c, p);
+    att[1] = lutValue(c + 2, p);
+    att[2] = lutValue(c + 4, p);
+}
+
 template <typename T>
 __device__ inline void attenuation(const Tables<T>& t, uint32_t mat, T E, T att[3])
 {
-    const T le = Lut<T>::logE(E);
-    const uint32_t k = lutInterval(t, le);
-    const T anchor = t.lutGrid[k];
-    const T* c = t.lutCoef + (static_cast<size_t>(mat) * t.lutNodes + k) * 6;
-    att[0] = Lut<T>::value(c[0], c[1], le, anchor);
-    att[1] = Lut<T>::value(c[2], c[3], le, anchor);
-    att[2] = Lut<T>::value(c[4], c[5], le, anchor);
+    attenuationAt(t, mat, lutPoint(t, E), att);
+}
+
+// 1 / majorant (maxAttenuationInverse, attenuationinterpolator.hpp:231-248)
+template <typename T>
+__device__ inline T majorantInverseAt(const Tables<T>& t, const LutPoint<T>& p)
+{
+    return lutValue(t.lutMajorant + p.k * 2, p);
 }
 
 template <typename T>
 __device__ inline T majorantInverse(const Tables<T>& t, T E)
 {
-    const T le = Lut<T>::logE(E);
-    const uint32_t k = lutInterval(t, le);
-    const T* c = t.lutMajorant + k * 2;
-    return Lut<T>::value(c[0], c[1], le, t.lutGrid[k]);
+    return majorantInverseAt(t, lutPoint(t, E));
 }
 
 // RITA sample of q^2 below qsqrMax (dxmcrandom.hpp:483-500)

@@ -310,6 +310,14 @@ int dxmc_hip_ctx_create(int device, dxmc_ctx** out)
         delete c;
         return DXMC_ERR_DEVICE;
     }
+    // keep memory freed by hipFreeAsync in the device pool for reuse instead
+    // of returning it at every synchronisation
+    hipMemPool_t pool = nullptr;
+    if (hipDeviceGetDefaultMemPool(&pool, device) == hipSuccess) {
+        uint64_t keep = UINT64_MAX;
+        hipMemPoolSetAttribute(pool, hipMemPoolAttrReleaseThreshold, &keep);
+    }
+    hipGetLastError();
     hipDeviceProp_t prop;
     if (hipGetDeviceProperties(&prop, device) == hipSuccess) {
         c->numCU = prop.multiProcessorCount;
@@ -453,11 +461,12 @@ int dxmc_hip_world_destroy(dxmc_ctx* c, dxmc_world* w)
     std::erase_if(c->tuned, [w](const auto& kv) { return std::get<0>(kv.first) == w; });
     hipSetDevice(c->device);
     hipStreamSynchronize(c->stream);
-    hipFree(w->material);
-    hipFree(w->density);
-    hipFree(w->measurement);
-    hipFree(w->voxels);
-    hipFree(w->palette);
+    c->release(w->material);
+    c->release(w->density);
+    c->release(w->measurement);
+    c->release(w->voxels);
+    c->release(w->palette);
+    hipStreamSynchronize(c->stream);
     delete w;
     return DXMC_OK;
 }
@@ -502,7 +511,7 @@ int dxmc_hip_tables_destroy(dxmc_ctx* c, dxmc_tables* t)
     hipSetDevice(c->device);
     hipStreamSynchronize(c->stream);
     for (void* p : t->allocations)
-        hipFree(p);
+        c->release(p);
     delete t;
     return DXMC_OK;
 }
@@ -539,9 +548,10 @@ int dxmc_hip_tally_destroy(dxmc_ctx* c, dxmc_tally* t)
         return c->fail(DXMC_ERR_STATE, "tally_destroy: tally belongs to another context");
     hipSetDevice(c->device);
     hipStreamSynchronize(c->stream);
-    hipFree(t->dose);
-    hipFree(t->nEvents);
-    hipFree(t->variance);
+    c->release(t->dose);
+    c->release(t->nEvents);
+    c->release(t->variance);
+    hipStreamSynchronize(c->stream);
     delete t;
     return DXMC_OK;
 }
@@ -564,11 +574,12 @@ int dxmc_hip_tally_download(dxmc_ctx* c, const dxmc_tally* t, double* dose, uint
     DXMC_CHECK(c, hipSetDevice(c->device));
     DXMC_CHECK(c, hipStreamSynchronize(c->stream));
     if (dose)
-        DXMC_CHECK(c, hipMemcpy(dose, t->dose, t->n * sizeof(double), hipMemcpyDeviceToHost));
+        DXMC_CHECK(c, hipMemcpyAsync(dose, t->dose, t->n * sizeof(double), hipMemcpyDeviceToHost, c->stream));
     if (nEvents)
-        DXMC_CHECK(c, hipMemcpy(nEvents, t->nEvents, t->n * sizeof(uint32_t), hipMemcpyDeviceToHost));
+        DXMC_CHECK(c, hipMemcpyAsync(nEvents, t->nEvents, t->n * sizeof(uint32_t), hipMemcpyDeviceToHost, c->stream));
     if (variance)
-        DXMC_CHECK(c, hipMemcpy(variance, t->variance, t->n * sizeof(double), hipMemcpyDeviceToHost));
+        DXMC_CHECK(c, hipMemcpyAsync(variance, t->variance, t->n * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    DXMC_CHECK(c, hipStreamSynchronize(c->stream));
     return DXMC_OK;
 }
 
@@ -670,7 +681,8 @@ int dxmc_hip_rng_streams(dxmc_ctx* c, uint64_t seed, uint64_t first, uint64_t n,
     hipLaunchKernelGGL(rngStreams, dim3(gridFor(n, 256)), dim3(256), 0, c->stream, seed, first, n, draws, static_cast<uint32_t*>(buf.p));
     DXMC_CHECK(c, hipGetLastError());
     DXMC_CHECK(c, hipStreamSynchronize(c->stream));
-    DXMC_CHECK(c, hipMemcpy(out, buf.p, n * draws * sizeof(uint32_t), hipMemcpyDeviceToHost));
+    if (int e = c->download(out, buf.p, n * draws * sizeof(uint32_t)))
+        return e;
     return DXMC_OK;
 }
 
@@ -695,8 +707,10 @@ int dxmc_hip_lut_lookup(dxmc_ctx* c, const dxmc_tables* t, int32_t precision, ui
             static_cast<const double*>(de.p), static_cast<double*>(da.p), static_cast<double*>(dj.p));
     DXMC_CHECK(c, hipGetLastError());
     DXMC_CHECK(c, hipStreamSynchronize(c->stream));
-    DXMC_CHECK(c, hipMemcpy(outAtt, da.p, n * 24, hipMemcpyDeviceToHost));
-    DXMC_CHECK(c, hipMemcpy(outMaj, dj.p, n * 8, hipMemcpyDeviceToHost));
+    if (int e = c->download(outAtt, da.p, n * 24))
+        return e;
+    if (int e = c->download(outMaj, dj.p, n * 8))
+        return e;
     return DXMC_OK;
 }
 
@@ -723,13 +737,12 @@ int dxmc_hip_interact(dxmc_ctx* c, const dxmc_tables* t, int32_t precision, int3
             static_cast<double*>(de.p), static_cast<int32_t*>(da.p), tl, c->stream);
     DXMC_CHECK(c, hipGetLastError());
     DXMC_CHECK(c, hipStreamSynchronize(c->stream));
-    DXMC_CHECK(c, hipMemcpy(particles, dp.p, n * 64, hipMemcpyDeviceToHost));
-    DXMC_CHECK(c, hipMemcpy(aliveOut, da.p, n * 4, hipMemcpyDeviceToHost));
-    if (kind <= 2)
-        DXMC_CHECK(c, hipMemcpy(energyOut, de.p, n * 8, hipMemcpyDeviceToHost));
-    else
-        DXMC_CHECK(c, hipMemcpy(energyOut, td.p, n * 8, hipMemcpyDeviceToHost));
-    return DXMC_OK;
+    if (int e = c->download(particles, dp.p, n * 64))
+        return e;
+    if (int e = c->download(aliveOut, da.p, n * 4))
+        return e;
+    // kinds 0-2 return the energy deposited, 3-4 the tally the dispatch scored into
+    return c->download(energyOut, kind <= 2 ? de.p : td.p, n * 8);
 }
 
 int dxmc_hip_sample_particles(dxmc_ctx* c, const dxmc_world* w, const dxmc_tables* t, int32_t precision, const dxmc_exposure* x, uint64_t seed,
@@ -752,8 +765,10 @@ int dxmc_hip_sample_particles(dxmc_ctx* c, const dxmc_world* w, const dxmc_table
             first, n, static_cast<double*>(dp.p), static_cast<int32_t*>(di.p));
     DXMC_CHECK(c, hipGetLastError());
     DXMC_CHECK(c, hipStreamSynchronize(c->stream));
-    DXMC_CHECK(c, hipMemcpy(particles, dp.p, n * 64, hipMemcpyDeviceToHost));
-    DXMC_CHECK(c, hipMemcpy(inside, di.p, n * 4, hipMemcpyDeviceToHost));
+    if (int e = c->download(particles, dp.p, n * 64))
+        return e;
+    if (int e = c->download(inside, di.p, n * 4))
+        return e;
     return DXMC_OK;
 }
 
@@ -835,7 +850,8 @@ int runTransport(dxmc_ctx* c, const dxmc_world* w, const dxmc_tables* t, const d
     }
     if (stats) {
         Counters h {};
-        DXMC_CHECK(c, hipMemcpy(&h, rb.counters.p, sizeof(Counters), hipMemcpyDeviceToHost));
+        if (int e = c->download(&h, rb.counters.p, sizeof(Counters)))
+        return e;
         stats->histories = total;
         stats->steps = h.steps;
         stats->events = h.events;

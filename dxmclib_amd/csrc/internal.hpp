@@ -36,11 +36,15 @@ struct dxmc_ctx {
         error = std::string("HIP error: ") + hipGetErrorString(e);
         return DXMC_ERR_DEVICE;
     }
+    // Device memory is stream-ordered (hipMallocAsync / hipFreeAsync on the
+    // context's stream, from the device's pool, which keeps freed blocks):
+    // unlike hipFree, releasing memory then does not wait for the kernels
+    // other contexts run on the same device.
     template <typename P>
     int alloc(P** p, size_t bytes)
     {
         void* q = nullptr;
-        const hipError_t e = hipMalloc(&q, bytes ? bytes : 1);
+        const hipError_t e = hipMallocAsync(&q, bytes ? bytes : 1, stream);
         if (e != hipSuccess) {
             hipGetLastError();
             error = "device allocation of " + std::to_string(bytes) + " bytes failed";
@@ -49,11 +53,28 @@ struct dxmc_ctx {
         *p = static_cast<P*>(q);
         return DXMC_OK;
     }
+    void release(void* p)
+    {
+        if (p)
+            hipFreeAsync(p, stream);
+    }
+    // host -> device, complete on return (the host array may be freed)
     int copy(void* dst, const void* src, size_t bytes)
     {
         if (bytes == 0)
             return DXMC_OK;
-        return check(hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice));
+        if (int st = check(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, stream)))
+            return st;
+        return check(hipStreamSynchronize(stream));
+    }
+    // device -> host, complete on return
+    int download(void* dst, const void* src, size_t bytes)
+    {
+        if (bytes == 0)
+            return DXMC_OK;
+        if (int st = check(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, stream)))
+            return st;
+        return check(hipStreamSynchronize(stream));
     }
 };
 
@@ -111,19 +132,13 @@ struct DeviceBuffer {
         : c(ctx)
     {
     }
-    ~DeviceBuffer()
-    {
-        if (p)
-            hipFree(p);
-    }
+    ~DeviceBuffer() { c->release(p); }
     DeviceBuffer(const DeviceBuffer&) = delete;
     DeviceBuffer& operator=(const DeviceBuffer&) = delete;
     int alloc(size_t bytes)
     {
-        if (p) {
-            hipFree(p);
-            p = nullptr;
-        }
+        c->release(p);
+        p = nullptr;
         return c->alloc(&p, bytes);
     }
     int upload(const void* src, size_t bytes)
@@ -136,7 +151,7 @@ struct DeviceBuffer {
     {
         if (int st = alloc(bytes))
             return st;
-        return c->check(hipMemset(p, 0, bytes));
+        return c->check(hipMemsetAsync(p, 0, bytes, c->stream));
     }
 };
 

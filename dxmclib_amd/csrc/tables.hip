@@ -4,9 +4,10 @@
 // FP32 copy: the attenuation LUT is re-based from log10 to log2 and anchored
 // at each segment's upper grid node, so the kernels evaluate
 // mu = exp2(y_k + a (log2 E - x_k)) with y_k = log2 mu(x_k) (no cancellation
-// for steep segments); grid and linear-index parameters are scaled by
-// log2(10); all other tables are rounded to float. The conversion is done in
-// double on the host.
+// for steep segments). The segment search runs on the FP64 grid, corrected
+// by per-segment energy bounds that a small kernel derives from the FP64
+// copy, so both paths pick the same segment. All other tables are rounded
+// to float. The conversion is done in double on the host.
 #include "internal.hpp"
 
 #include <cmath>
@@ -81,6 +82,39 @@ bool validateTables(const dxmc_tables_desc& d, std::string& why)
 }
 
 namespace {
+    // lowest FP64 energy that the FP64 path places in segment k or above
+    // (bisection over the ordered bit patterns of positive doubles, with the
+    // FP64 path's own log10 and segment search)
+    __global__ void segmentBounds(dxmc_dev::Tables<double> t, double* bound)
+    {
+        const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+        if (k >= t.lutNodes)
+            return;
+        if (k == 0) {
+            bound[0] = 0.0;
+            return;
+        }
+        auto segment = [&](uint64_t bits) { return dxmc_dev::lutInterval(t, log10(__longlong_as_double(static_cast<long long>(bits)))); };
+        uint64_t lo = static_cast<uint64_t>(__double_as_longlong(1e-6));
+        uint64_t hi = static_cast<uint64_t>(__double_as_longlong(1e6));
+        if (segment(hi) < k) {
+            bound[k] = HUGE_VAL;
+            return;
+        }
+        if (segment(lo) >= k) {
+            bound[k] = 1e-6;
+            return;
+        }
+        while (hi - lo > 1) {
+            const uint64_t mid = lo + (hi - lo) / 2;
+            if (segment(mid) >= k)
+                hi = mid;
+            else
+                lo = mid;
+        }
+        bound[k] = __longlong_as_double(static_cast<long long>(hi));
+    }
+
     template <typename T, typename S>
     int upload(dxmc_ctx* c, dxmc_tables& t, const std::vector<S>& host, const T** dst)
     {
@@ -107,12 +141,27 @@ namespace {
         out.nMaterials = d.n_materials;
         out.lutNodes = d.lut_nodes;
         out.lutLinearIndex = d.lut_linear_index;
-        out.lutLinearStep = static_cast<T>(d.lut_linear_step * ls);
-        out.lutLinearEnergy = static_cast<T>(d.lut_linear_energy * ls);
+        out.lutLinearStep = d.lut_linear_step;
+        out.lutLinearEnergy = d.lut_linear_energy;
         out.nSpectra = d.n_spectra;
         int st;
-        if ((st = upload(c, t, cast<T>(d.lut_grid, d.lut_nodes, ls), &out.lutGrid)))
+        if ((st = upload(c, t, cast<double>(d.lut_grid, d.lut_nodes), &out.lutGrid)))
             return st;
+        out.lutBound = nullptr;
+        out.lutAnchor = nullptr;
+        if (base2) {
+            // segment bounds from the FP64 copy (built first), float anchors
+            double* bound = nullptr;
+            if ((st = c->alloc(&bound, d.lut_nodes * sizeof(double))))
+                return st;
+            t.allocations.push_back(bound);
+            hipLaunchKernelGGL(segmentBounds, dim3((d.lut_nodes + 63) / 64), dim3(64), 0, c->stream, t.f64, bound);
+            if ((st = c->check(hipGetLastError())) || (st = c->check(hipStreamSynchronize(c->stream))))
+                return st;
+            out.lutBound = bound;
+            if ((st = upload(c, t, cast<float>(d.lut_grid, d.lut_nodes, LOG2_10), &out.lutAnchor)))
+                return st;
+        }
         // FP64: (b, a) as built; FP32: (log2 mu at the upper node x_k, a)
         auto pair = [&](const double* ba, size_t node, T* out2) {
             if (base2) {

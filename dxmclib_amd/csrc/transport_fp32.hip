@@ -55,18 +55,6 @@ __device__ inline uint64_t voxelIndexFast(const World<float>& w, const float* po
     return (static_cast<uint64_t>(iz) * w.ny + iy) * w.nx + ix;
 }
 
-// Position of an energy in the FP32 LUT: segment k and the offset of
-// log2 E from the segment's anchor node. It changes only with the photon's
-// energy, so the kernel keeps it per photon and each flight evaluates
-// mu = 2^(y_k + a_k * dl) for the voxel's material without the log and the
-// segment search (same arithmetic as attenuation() / majorantInverse()).
-__device__ inline void lutPoint(const Tables<float>& t, float E, uint32_t& k, float& dl)
-{
-    const float le = Lut<float>::logE(E);
-    k = lutInterval(t, le);
-    dl = le - t.lutGrid[k];
-}
-
 // Russian roulette after a step that left the photon alive (transport.hpp:684-694)
 __device__ inline bool roulette(Particle<float>& p, Rng& rng)
 {
@@ -103,8 +91,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(W, 8))) voi
     uint64_t idx = 0;
     uint32_t mat = 0, meas = 0;
     float density = 0;
-    uint32_t lutK = 0;
-    float lutDl = 0;
+    // the photon's place in the LUT; changes only with its energy
+    LutPoint<float> lp { 0, 0.0f };
     const float inv[3] = { 1.0f / w.spacing[0], 1.0f / w.spacing[1], 1.0f / w.spacing[2] };
     const uint32_t lane = __lane_id();
 
@@ -163,10 +151,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(W, 8))) voi
         if (nPend != 0 && (nPend >= interactThreshold || fly == 0)) {
             if (phase == PENDING) {
                 float att[3];
-                const float* cf = t.lutCoef + (static_cast<size_t>(mat) * t.lutNodes + lutK) * 6;
-                att[0] = exp2f(cf[0] + cf[1] * lutDl);
-                att[1] = exp2f(cf[2] + cf[3] * lutDl);
-                att[2] = exp2f(cf[4] + cf[5] * lutDl);
+                attenuationAt(t, mat, lp, att);
                 const float prob = ((att[0] + att[1]) + att[2]) * density * majInv;
                 bool alive = meas == 0 ? computeInteractions<float, L>(t, att, p, mat, sc, idx, rng, update)
                                        : computeInteractionsForced<float, L>(t, prob, att, p, mat, sc, idx, rng, update);
@@ -176,9 +161,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(W, 8))) voi
             }
         } else if (phase == FLYING) {
             if (update) {
-                lutPoint(t, p.energy, lutK, lutDl);
-                const float* m = t.lutMajorant + lutK * 2;
-                majInv = exp2f(m[0] + m[1] * lutDl);
+                lp = lutPoint(t, p.energy);
+                majInv = majorantInverseAt(t, lp);
                 update = false;
             }
             const float step = -logf(rng.uniform<float>()) * majInv * 10.0f; // cm -> mm
@@ -191,10 +175,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(W, 8))) voi
                 idx = voxelIndexFast(w, p.pos, inv);
                 readVoxel(w, idx, mat, density, meas);
                 float att[3];
-                const float* cf = t.lutCoef + (static_cast<size_t>(mat) * t.lutNodes + lutK) * 6;
-                att[0] = exp2f(cf[0] + cf[1] * lutDl);
-                att[1] = exp2f(cf[2] + cf[3] * lutDl);
-                att[2] = exp2f(cf[4] + cf[5] * lutDl);
+                attenuationAt(t, mat, lp, att);
                 const float prob = ((att[0] + att[1]) + att[2]) * density * majInv;
                 if (meas == 0) {
                     if (rng.uniform<float>() < prob)

@@ -118,29 +118,36 @@ public:
     const dxmc_run_stats& lastRunStats() const { return m_stats; }
     const std::string& lastError() const { return m_error; }
 
+    // The host part of operator(): attenuation tables for the world's
+    // materials, the source's exposure table aligned to the world basis and
+    // the FP64 density copy handed to the device.
+    struct Prepared {
+        bool valid = false;
+        std::vector<double> density;
+        TableSet tables;
+        std::vector<dxmc_exposure> exposures;
+        std::uint64_t histories = 0;
+    };
+
     template <typename U>
         requires std::is_base_of<World<T>, U>::value
-    Result<T> operator()(const U& world, Source<T>* source, ProgressBar<T>* progressbar = nullptr, bool useSourceDoseCalibration = true)
+    Prepared prepare(const U& world, Source<T>* source)
     {
-        Result<T> result(world.size());
-        m_error.clear();
+        Prepared p;
         if (!world.isValid() || !source)
-            return result;
+            return p;
         source->updateFromWorld(world);
         source->validate();
         if (!source->isValid())
-            return result;
-        result.numberOfHistories = source->historiesPerExposure() * source->totalExposures();
-
+            return p;
+        p.histories = source->historiesPerExposure() * source->totalExposures();
         // host tables (always FP64; the device derives its FP32 copy)
-        std::vector<double> dens(world.densityArray()->cbegin(), world.densityArray()->cend());
-        const auto maxDens = AttenuationLutInterpolator<double>::maxDensities(world.materialMap().size(), dens.cbegin(), dens.cend(),
+        p.density.assign(world.densityArray()->cbegin(), world.densityArray()->cend());
+        const auto maxDens = AttenuationLutInterpolator<double>::maxDensities(world.materialMap().size(), p.density.cbegin(), p.density.cend(),
             world.materialIndexArray()->cbegin());
         m_attenuationLut.generate(world.materialMap(), maxDens, static_cast<double>(source->maxPhotonEnergyProduced()), 1.0);
-
-        TableSet tables;
-        tables.setMaterialTables(m_attenuationLut);
-        ExposureTable<T> exposures(tables);
+        p.tables.setMaterialTables(m_attenuationLut);
+        ExposureTable<T> exposures(p.tables);
         const auto& basis = world.directionCosines();
         const std::uint64_t nExp = source->totalExposures();
         for (std::uint64_t i = 0; i < nExp; ++i) {
@@ -148,8 +155,26 @@ public:
             e.alignToDirectionCosines(basis);
             exposures.add(e);
         }
+        p.exposures = exposures.records();
+        p.valid = true;
+        return p;
+    }
+
+    // The device part of operator() on a prepared run. 'context' selects the
+    // HIP context (and so the stream) to run on; nullptr uses the device's
+    // shared context.
+    template <typename U>
+        requires std::is_base_of<World<T>, U>::value
+    Result<T> execute(const U& world, Source<T>* source, const Prepared& prep, ProgressBar<T>* progressbar = nullptr,
+        bool useSourceDoseCalibration = true, dxmc_ctx* context = nullptr)
+    {
+        Result<T> result(world.size());
+        m_error.clear();
+        if (!prep.valid)
+            return result;
+        result.numberOfHistories = prep.histories;
         if (progressbar) {
-            progressbar->setTotalExposures(nExp);
+            progressbar->setTotalExposures(prep.exposures.size());
             progressbar->setDoseData(result.dose.data(), world.dimensions(), world.spacing());
         }
         const auto start = std::chrono::steady_clock::now();
@@ -161,8 +186,8 @@ public:
                 return static_cast<double>(source->getCalibrationValue(m_lowenergyCorrection, progressbar));
             return 1.0;
         };
-        const bool ok = runOnDevice(world, dens, tables, exposures.records(), progressbar, calibration, dose, result.nEvents, variance,
-            result.numberOfHistories);
+        const bool ok = runOnDevice(world, prep.density, prep.tables, prep.exposures, progressbar, calibration, dose, result.nEvents, variance,
+            result.numberOfHistories, context);
         result.simulationTime = std::chrono::steady_clock::now() - start;
         if (progressbar)
             progressbar->clearDoseData();
@@ -182,15 +207,27 @@ public:
         return result;
     }
 
+    // Transport<T>::operator() of the reference (transport.hpp:135-202)
+    template <typename U>
+        requires std::is_base_of<World<T>, U>::value
+    Result<T> operator()(const U& world, Source<T>* source, ProgressBar<T>* progressbar = nullptr, bool useSourceDoseCalibration = true)
+    {
+        m_error.clear();
+        const Prepared prep = prepare(world, source);
+        if (!prep.valid)
+            return Result<T>(world.size());
+        return execute(world, source, prep, progressbar, useSourceDoseCalibration);
+    }
+
     const AttenuationLut<double>& attenuationLut() const { return m_attenuationLut; }
 
 protected:
     template <typename Calibration>
     bool runOnDevice(const World<T>& world, const std::vector<double>& dens, const TableSet& tables, const std::vector<dxmc_exposure>& exposures,
         ProgressBar<T>* pb, Calibration calibration, std::vector<double>& dose, std::vector<std::uint32_t>& nEvents, std::vector<double>& variance,
-        std::uint64_t nHist)
+        std::uint64_t nHist, dxmc_ctx* context)
     {
-        dxmc_ctx* c = DeviceContexts::get(m_device, m_error);
+        dxmc_ctx* c = context ? context : DeviceContexts::get(m_device, m_error);
         if (!c)
             return false;
         dxmc_world_desc wd {};

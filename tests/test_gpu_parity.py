@@ -228,7 +228,8 @@ def test_fp32_voxel_records(device, records):
     FP64 path on identical streams."""
     print(f"voxel records {records}", flush=True)
     rng = np.random.default_rng(5)
-    shape = (40, 40, 40)
+    # the 8-byte case needs more than 65536 distinct records
+    shape = (48, 48, 48) if records == "direct" else (40, 40, 40)
     mat = (rng.random(shape) < 0.3).astype(np.uint8)
     base = np.array([1.0, 1.92], dtype=np.float32)[mat]
     if records == "u8":
@@ -240,12 +241,13 @@ def test_fp32_voxel_records(device, records):
         distinct, vb = np.unique(np.stack([dens.ravel().view(np.uint32), mat.ravel().astype(np.uint32)]), axis=1).shape[1], 2
     else:
         dens = (base * rng.uniform(0.98, 1.02, shape)).astype(np.float32)
+        assert np.unique(np.stack([dens.ravel().view(np.uint32), mat.ravel().astype(np.uint32)]), axis=1).shape[1] > 65536
         distinct, vb = 0, 8
     # pencil through voxel centres (a beam along voxel faces lets the FP32
     # and FP64 paths pick different voxels for photons within an FP32 ulp of
     # the face, so their histories decorrelate)
-    spec = backend.source_spec(capi.SOURCE_PENCIL, position=(0.5, 0.5, -40.0), direction_cosines=(1, 0, 0, 0, 1, 0), photon_energy=60.0,
-                               histories_per_exposure=50_000, total_exposures=4)
+    spec = backend.source_spec(capi.SOURCE_PENCIL, position=(0.5, 0.5, -2.0 * shape[0]), direction_cosines=(1, 0, 0, 0, 1, 0),
+                               photon_energy=60.0, histories_per_exposure=50_000, total_exposures=4)
     wl = workloads.Workload("records", mat, dens, (1.0, 1.0, 1.0), ["Water, Liquid", "Bone, Cortical (ICRU)"], spec, 200_000)
     ext = backend.world_extent(shape[::-1], wl.spacing)
     w = device.upload_world(mat, dens, wl.spacing, ext, 2)
