@@ -353,3 +353,65 @@ def transport(world, spec, precision=capi.PRECISION_FP64, low_energy_correction=
     _check_host(capi.host().dxmc_host_transport(C.byref(world), C.byref(spec), C.byref(cfg), _ptr(dose, C.c_double), _ptr(ev, C.c_uint32),
                                                 _ptr(var, C.c_double), C.byref(stats), C.byref(secs), C.byref(nh)))
     return {"dose": dose, "n_events": ev, "variance": var, "stats": stats, "seconds": secs.value, "histories": nh.value}
+
+
+class Server:
+    """Request server over one GPU (dxmc_host_server_*): requests are
+    prepared on host threads, admitted to device slots by memory and run
+    concurrently; results and per-request timing are collected by id."""
+
+    def __init__(self, device=0, precision=capi.PRECISION_FP32, low_energy_correction=1, slots=4, prep_threads=4, memory_budget=0,
+                 tuning=None):
+        cfg = capi.ServerCfg()
+        cfg.device = device
+        cfg.precision = precision
+        cfg.low_energy_correction = low_energy_correction
+        cfg.slots = slots
+        cfg.prep_threads = prep_threads
+        cfg.memory_budget = memory_budget
+        if tuning is not None:
+            cfg.tuning = tuning
+        self.lib = capi.host()
+        h = C.c_void_p()
+        _check_host(self.lib.dxmc_host_server_create(C.byref(cfg), C.byref(h)))
+        self.h = h
+        self.sizes = {}
+
+    def close(self):
+        if self.h:
+            self.lib.dxmc_host_server_destroy(self.h)
+            self.h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def clock(self):
+        return self.lib.dxmc_host_server_clock(self.h)
+
+    def pause(self, paused=True):
+        _check_host(self.lib.dxmc_host_server_pause(self.h, 1 if paused else 0))
+
+    def submit(self, world, spec, seed=42, output_mode=capi.OUTPUT_EV_PER_HISTORY):
+        """world: a HostWorld (host_world()); spec: a SourceSpec."""
+        rid = C.c_uint64()
+        _check_host(self.lib.dxmc_host_server_submit(self.h, C.byref(world), C.byref(spec), seed, output_mode, C.byref(rid)))
+        self.sizes[rid.value] = world.dims[0] * world.dims[1] * world.dims[2]
+        return rid.value
+
+    def wait(self, rid, timeout=-1.0):
+        st = C.c_int32()
+        _check_host(self.lib.dxmc_host_server_wait(self.h, rid, timeout, C.byref(st)))
+        return st.value
+
+    def collect(self, rid):
+        n = self.sizes.pop(rid)
+        dose = np.zeros(n)
+        ev = np.zeros(n, dtype=np.uint32)
+        var = np.zeros(n)
+        info = capi.RequestInfo()
+        _check_host(self.lib.dxmc_host_server_collect(self.h, rid, _ptr(dose, C.c_double), _ptr(ev, C.c_uint32), _ptr(var, C.c_double),
+                                                      C.byref(info)))
+        return {"dose": dose, "n_events": ev, "variance": var, "info": info}

@@ -37,7 +37,7 @@ HIP_SOURCES = {
     "transport_fp32.hip": [],
     "autotune.hip": [],
 }
-HOST_SOURCES = ["src/material.cpp", "src/host_capi.cpp"]
+HOST_SOURCES = ["src/material.cpp", "src/host_capi.cpp", "src/server.cpp"]
 HOST_FLAGS = ["-O2", "-std=c++20", "-fPIC", "-ffp-contract=off",
               "-I", os.path.join(HERE, "include"), "-I", os.path.join(ROOT, "include")]
 

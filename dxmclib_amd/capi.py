@@ -138,7 +138,7 @@ class RunCfg(C.Structure):
         ("interaction_threshold", C.c_uint32),
         ("queue_depth", C.c_uint32),
         ("waves_per_simd", C.c_uint32),
-        ("reserved", C.c_uint32),
+        ("lds_tables", C.c_uint32),
     ]
 
 
@@ -229,6 +229,40 @@ class HostTransportCfg(C.Structure):
     ]
 
 
+class ServerCfg(C.Structure):
+    _fields_ = [
+        ("device", C.c_int32),
+        ("precision", C.c_int32),
+        ("low_energy_correction", C.c_int32),
+        ("slots", C.c_uint32),
+        ("prep_threads", C.c_uint32),
+        ("reserved", C.c_uint32),
+        ("memory_budget", C.c_uint64),
+        ("tuning", RunCfg),
+    ]
+
+
+class RequestInfo(C.Structure):
+    _fields_ = [
+        ("submitted", C.c_double),
+        ("prepared", C.c_double),
+        ("admitted", C.c_double),
+        ("finished", C.c_double),
+        ("histories", C.c_uint64),
+        ("steps", C.c_uint64),
+        ("events", C.c_uint64),
+        ("device_bytes", C.c_uint64),
+        ("status", C.c_int32),
+        ("slot", C.c_int32),
+    ]
+
+
+REQUEST_QUEUED = 0
+REQUEST_RUNNING = 1
+REQUEST_DONE = 2
+REQUEST_UNKNOWN = 3
+REQUEST_READY = 4
+
 # every symbol include/dxmc_hip.h declares, with its ctypes signature
 HIP_SYMBOLS = {
     "dxmc_hip_abi_version": (C.c_int, []),
@@ -270,6 +304,14 @@ HOST_SYMBOLS = {
     "dxmc_host_transport": (C.c_int, [C.POINTER(HostWorld), C.POINTER(SourceSpec), C.POINTER(HostTransportCfg), f64p, u32p, f64p,
                                       C.POINTER(RunStats), f64p, C.POINTER(C.c_uint64)]),
     "dxmc_host_material_data": (C.c_int, [C.c_char_p, C.c_double, f64p, f64p]),
+    "dxmc_host_server_create": (C.c_int, [C.POINTER(ServerCfg), C.POINTER(C.c_void_p)]),
+    "dxmc_host_server_destroy": (C.c_int, [C.c_void_p]),
+    "dxmc_host_server_submit": (C.c_int, [C.c_void_p, C.POINTER(HostWorld), C.POINTER(SourceSpec), C.c_uint64, C.c_int32,
+                                          C.POINTER(C.c_uint64)]),
+    "dxmc_host_server_wait": (C.c_int, [C.c_void_p, C.c_uint64, C.c_double, C.POINTER(C.c_int32)]),
+    "dxmc_host_server_collect": (C.c_int, [C.c_void_p, C.c_uint64, f64p, u32p, f64p, C.POINTER(RequestInfo)]),
+    "dxmc_host_server_pause": (C.c_int, [C.c_void_p, C.c_int32]),
+    "dxmc_host_server_clock": (C.c_double, [C.c_void_p]),
 }
 
 _libs = {}

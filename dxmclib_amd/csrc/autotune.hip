@@ -96,13 +96,15 @@ int autotune(dxmc_ctx* c, const dxmc_world* w, const dxmc_tables* t, const dxmc_
     cfg.flags &= ~DXMC_RUN_AUTOTUNE;
     dxmc_run_cfg defaults = cfg;
     defaults.block_size = defaults.blocks_per_cu = defaults.regeneration_threshold = defaults.interaction_threshold = defaults.queue_depth = 0;
-    defaults.waves_per_simd = 0;
+    defaults.waves_per_simd = defaults.lds_tables = 0;
     double defaultRate = tr.rate(defaults, st);
     double bestRate = st ? -1 : tr.rate(cfg, st);
     if (!st) {
         if (cfg.precision == DXMC_PRECISION_FP32) {
             st = searchField(tr, cfg, &dxmc_run_cfg::waves_per_simd, { 6, 8 }, bestRate);
-            const uint32_t resident = residentBlocksFp32(cfg.low_energy_correction, cfg.waves_per_simd);
+            if (!st)
+                st = searchField(tr, cfg, &dxmc_run_cfg::lds_tables, { 1, 2, 3 }, bestRate);
+            const uint32_t resident = residentBlocksFp32(cfg.low_energy_correction, cfg.waves_per_simd, 0);
             std::vector<uint32_t> perCU;
             for (uint32_t b : { resident > 1 ? resident - 1 : 1u, std::max(1u, resident / 2) })
                 if (b != resident && std::find(perCU.begin(), perCU.end(), b) == perCU.end())

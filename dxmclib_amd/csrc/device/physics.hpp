@@ -71,15 +71,17 @@ struct Heel {
     uint32_t energySize, angleSize, weightOffset, pad;
 };
 
-// Flat device tables (one copy per precision). For T = float the LUT grid and
-// linear step/energy are in log2 units and each coefficient pair is
-// (log2 mu at the segment's upper node, slope).
+// Flat device tables (one copy per precision). The LUT grid and linear
+// step/energy are log10 E in both copies; for T = float each coefficient pair
+// is (log2 mu at the segment's upper node, slope).
 template <typename T>
 struct Tables {
     uint32_t nMaterials;
     uint32_t lutNodes;
     uint32_t lutLinearIndex;
     uint32_t nSpectra;
+    // element counts of the variable-length tables
+    uint32_t specTotal, nHeel, heelWeightTotal, nBowtie, bowtieTotal, pad;
     double lutLinearStep; // log10 units, both precisions
     double lutLinearEnergy;
     const double* lutGrid; // log10 E of the nodes, both precisions

@@ -629,7 +629,7 @@ int dxmc_hip_run(dxmc_ctx* c, const dxmc_world* w, const dxmc_tables* t, const d
         }
         const dxmc_run_cfg& b = it->second;
         for (auto f : { &dxmc_run_cfg::block_size, &dxmc_run_cfg::blocks_per_cu, &dxmc_run_cfg::regeneration_threshold,
-                 &dxmc_run_cfg::interaction_threshold, &dxmc_run_cfg::queue_depth, &dxmc_run_cfg::waves_per_simd })
+                 &dxmc_run_cfg::interaction_threshold, &dxmc_run_cfg::queue_depth, &dxmc_run_cfg::waves_per_simd, &dxmc_run_cfg::lds_tables })
             if (run.*f == 0)
                 run.*f = b.*f;
     }

@@ -212,6 +212,7 @@ namespace {
             thr[s] = static_cast<uint32_t>((0ULL - n) % n);
         }
         const uint32_t total = ns ? d.spectrum_offset[ns] : 1;
+        out.specTotal = total;
         prob = ns ? cast<T>(d.spectrum_probability, total) : std::vector<T>(1, T(1));
         energy = ns ? cast<T>(d.spectrum_energy, total) : std::vector<T>(1, T(0));
         alias = ns ? std::vector<uint32_t>(d.spectrum_alias, d.spectrum_alias + total) : std::vector<uint32_t>(1, 0);
@@ -227,6 +228,8 @@ namespace {
             nhw = std::max<size_t>(nhw, size_t(s.weight_offset) + size_t(s.energy_size) * s.angle_size);
         }
         const std::vector<T> hw = d.n_heel ? cast<T>(d.heel_weight, nhw) : std::vector<T>(1, T(1));
+        out.nHeel = static_cast<uint32_t>(heel.size());
+        out.heelWeightTotal = static_cast<uint32_t>(nhw);
         if ((st = upload(c, t, heel, &out.heel)) || (st = upload(c, t, hw, &out.heelWeight)))
             return st;
         const uint32_t nb = d.n_bowtie;
@@ -234,6 +237,8 @@ namespace {
         for (uint32_t b = 0; b <= nb && nb; ++b)
             boff[b] = d.bowtie_offset[b];
         const uint32_t btotal = nb ? d.bowtie_offset[nb] : 1;
+        out.nBowtie = nb;
+        out.bowtieTotal = btotal;
         const std::vector<T> ba = nb ? cast<T>(d.bowtie_angle, btotal) : std::vector<T>(1, T(0));
         const std::vector<T> bw = nb ? cast<T>(d.bowtie_weight, btotal) : std::vector<T>(1, T(1));
         if ((st = upload(c, t, boff, &out.bowtieOffset)) || (st = upload(c, t, ba, &out.bowtieAngle)) || (st = upload(c, t, bw, &out.bowtieWeight)))

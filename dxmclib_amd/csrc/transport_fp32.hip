@@ -55,6 +55,82 @@ __device__ inline uint64_t voxelIndexFast(const World<float>& w, const float* po
     return (static_cast<uint64_t>(iz) * w.ny + iy) * w.nx + ix;
 }
 
+// ------------------------------------------------------ LDS-staged tables
+// The tables the samplers read with data-dependent indices (RITA, spline,
+// shells, spectrum alias tables, heel and bowtie filters, the majorant and
+// the segment bounds, optionally the attenuation coefficients) are copied
+// into LDS at kernel start, one copy per block: their lookups then cost LDS
+// latency instead of L2 latency, also while the world gathers keep evicting
+// them from L1. Staging level: 1 none, 2 all but the attenuation
+// coefficients, 3 all.
+enum : uint32_t { STAGE_NONE = 1, STAGE_SMALL = 2, STAGE_ALL = 3 };
+
+__host__ __device__ constexpr uint32_t ldsAlign(uint64_t bytes) { return static_cast<uint32_t>((bytes + 15) & ~uint64_t(15)); }
+
+// bytes of LDS a staging level needs (the same walk as stageTables)
+__host__ __device__ inline uint32_t ldsBytes(const Tables<float>& t, uint32_t level)
+{
+    if (level <= STAGE_NONE)
+        return 0;
+    uint32_t b = 0;
+    b += ldsAlign(uint64_t(t.nMaterials) * 4 * 56 * sizeof(float));
+    b += ldsAlign(uint64_t(t.nMaterials) * 63 * sizeof(float));
+    b += ldsAlign(uint64_t(t.nMaterials) * 12 * sizeof(Shell<float>));
+    b += ldsAlign(uint64_t(t.lutNodes) * 2 * sizeof(float));
+    b += ldsAlign(uint64_t(t.lutNodes) * sizeof(double));
+    b += ldsAlign(uint64_t(t.lutNodes) * sizeof(float));
+    b += ldsAlign(uint64_t(t.nSpectra + 1) * sizeof(uint32_t));
+    b += ldsAlign(uint64_t(t.nSpectra ? t.nSpectra : 1) * sizeof(uint32_t));
+    b += 2 * ldsAlign(uint64_t(t.specTotal) * sizeof(float)) + ldsAlign(uint64_t(t.specTotal) * sizeof(uint32_t));
+    b += ldsAlign(uint64_t(t.nHeel) * sizeof(Heel<float>)) + ldsAlign(uint64_t(t.heelWeightTotal) * sizeof(float));
+    b += ldsAlign(uint64_t(t.nBowtie + 1) * sizeof(uint32_t)) + 2 * ldsAlign(uint64_t(t.bowtieTotal) * sizeof(float));
+    if (level >= STAGE_ALL)
+        b += ldsAlign(uint64_t(t.nMaterials) * t.lutNodes * 6 * sizeof(float));
+    return b;
+}
+
+template <typename T>
+__device__ inline const T* stageArray(const T* src, uint64_t count, unsigned char*& cursor)
+{
+    static_assert(sizeof(T) % 4 == 0);
+    T* dst = reinterpret_cast<T*>(cursor);
+    const uint64_t words = count * sizeof(T) / 4;
+    const uint32_t* s = reinterpret_cast<const uint32_t*>(src);
+    uint32_t* d = reinterpret_cast<uint32_t*>(dst);
+    for (uint64_t i = threadIdx.x; i < words; i += blockDim.x)
+        d[i] = s[i];
+    cursor += ldsAlign(count * sizeof(T));
+    return dst;
+}
+
+// copies the tables into LDS and points t at the copies (all threads of the
+// block call it; ends with a barrier)
+__device__ inline void stageTables(Tables<float>& t, uint32_t level, unsigned char* lds)
+{
+    if (level <= STAGE_NONE)
+        return;
+    unsigned char* c = lds;
+    t.rita = stageArray(t.rita, uint64_t(t.nMaterials) * 4 * 56, c);
+    t.spline = stageArray(t.spline, uint64_t(t.nMaterials) * 63, c);
+    t.shells = stageArray(t.shells, uint64_t(t.nMaterials) * 12, c);
+    t.lutMajorant = stageArray(t.lutMajorant, uint64_t(t.lutNodes) * 2, c);
+    t.lutBound = stageArray(t.lutBound, t.lutNodes, c);
+    t.lutAnchor = stageArray(t.lutAnchor, t.lutNodes, c);
+    t.specOffset = stageArray(t.specOffset, t.nSpectra + 1, c);
+    t.specThreshold = stageArray(t.specThreshold, t.nSpectra ? t.nSpectra : 1, c);
+    t.specProb = stageArray(t.specProb, t.specTotal, c);
+    t.specEnergy = stageArray(t.specEnergy, t.specTotal, c);
+    t.specAlias = stageArray(t.specAlias, t.specTotal, c);
+    t.heel = stageArray(t.heel, t.nHeel, c);
+    t.heelWeight = stageArray(t.heelWeight, t.heelWeightTotal, c);
+    t.bowtieOffset = stageArray(t.bowtieOffset, t.nBowtie + 1, c);
+    t.bowtieAngle = stageArray(t.bowtieAngle, t.bowtieTotal, c);
+    t.bowtieWeight = stageArray(t.bowtieWeight, t.bowtieTotal, c);
+    if (level >= STAGE_ALL)
+        t.lutCoef = stageArray(t.lutCoef, uint64_t(t.nMaterials) * t.lutNodes * 6, c);
+    __syncthreads();
+}
+
 // Russian roulette after a step that left the photon alive (transport.hpp:684-694)
 __device__ inline bool roulette(Particle<float>& p, Rng& rng)
 {
@@ -70,10 +146,13 @@ __device__ inline bool roulette(Particle<float>& p, Rng& rng)
 // (1: unconstrained, about 5 with this kernel; 6 and 8 trade a few spills
 // for occupancy). The autotuner picks among the variants.
 template <int L, int W>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(W, 8))) void transportPersistent(World<float> w, Tables<float> t, const Exposure<float>* exps, const uint64_t* expStart,
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(W, 8))) void transportPersistent(World<float> w, Tables<float> tg, const Exposure<float>* exps, const uint64_t* expStart,
     uint32_t nExp, uint64_t hBegin, uint64_t hEnd, uint64_t seed, uint64_t firstHistory, Tally tally, Counters* ctr, unsigned long long* next,
-    uint32_t regenThreshold, uint32_t interactThreshold, uint32_t poolSize)
+    uint32_t regenThreshold, uint32_t interactThreshold, uint32_t poolSize, uint32_t stage)
 {
+    extern __shared__ __align__(16) unsigned char lds[];
+    Tables<float> t = tg;
+    stageTables(t, stage, lds);
     Particle<float> p;
     Rng rng { 0, 1 };
     float majInv = 0;
@@ -201,7 +280,7 @@ namespace dxmc_backend {
 namespace {
 
 using KernelFn = void (*)(World<float>, Tables<float>, const Exposure<float>*, const uint64_t*, uint32_t, uint64_t, uint64_t, uint64_t, uint64_t,
-    Tally, Counters*, unsigned long long*, uint32_t, uint32_t, uint32_t);
+    Tally, Counters*, unsigned long long*, uint32_t, uint32_t, uint32_t, uint32_t);
 
 template <int L>
 KernelFn variant(uint32_t waves)
@@ -220,15 +299,38 @@ KernelFn kernelFor(int L, uint32_t waves)
 
 }
 
-uint32_t residentBlocksFp32(int L, uint32_t waves)
+uint32_t residentBlocksFp32(int L, uint32_t waves, uint32_t ldsPerBlock)
 {
     int resident = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&resident, reinterpret_cast<const void*>(kernelFor(L, waves)), 256, 0) != hipSuccess
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&resident, reinterpret_cast<const void*>(kernelFor(L, waves)), 256, ldsPerBlock) != hipSuccess
         || resident <= 0) {
         hipGetLastError();
-        resident = 4;
+        resident = 1;
     }
     return static_cast<uint32_t>(resident);
+}
+
+// staging level of a run: the requested one, or by default all tables when
+// they fit next to the register-limited number of resident blocks, else all
+// but the attenuation coefficients
+uint32_t stageLevelFp32(const dxmc_run_cfg& cfg, const Tables<float>& t)
+{
+    // a block gets at most 64 KiB of dynamic LDS
+    constexpr uint32_t ldsPerBlock = 64 * 1024;
+    auto fits = [&](uint32_t level) { return ldsBytes(t, level) <= ldsPerBlock; };
+    if (cfg.lds_tables >= STAGE_NONE && cfg.lds_tables <= STAGE_ALL) {
+        uint32_t level = cfg.lds_tables;
+        while (level > STAGE_NONE && !fits(level))
+            --level;
+        return level;
+    }
+    const uint32_t blocks = residentBlocksFp32(cfg.low_energy_correction, cfg.waves_per_simd, 0);
+    constexpr uint32_t ldsPerCU = 160 * 1024;
+    if (fits(STAGE_ALL) && uint64_t(ldsBytes(t, STAGE_ALL)) * blocks <= ldsPerCU)
+        return STAGE_ALL;
+    if (fits(STAGE_SMALL) && uint64_t(ldsBytes(t, STAGE_SMALL)) * blocks <= ldsPerCU)
+        return STAGE_SMALL;
+    return STAGE_NONE;
 }
 
 void launchFp32(dxmc_ctx* c, const dxmc_run_cfg& cfg, const World<float>& w, const Tables<float>& t, const Exposure<float>* exps,
@@ -238,16 +340,18 @@ void launchFp32(dxmc_ctx* c, const dxmc_run_cfg& cfg, const World<float>& w, con
     hipMemsetAsync(nx, 0, sizeof(unsigned long long), c->stream);
     const uint32_t block = 256;
     const uint32_t waves = cfg.waves_per_simd;
-    // default: as many blocks as stay resident on a CU (register-limited)
-    const uint32_t perCU = cfg.blocks_per_cu ? cfg.blocks_per_cu : residentBlocksFp32(cfg.low_energy_correction, waves);
+    const uint32_t stage = stageLevelFp32(cfg, t);
+    const uint32_t lds = ldsBytes(t, stage);
+    // default: as many blocks as stay resident on a CU (registers and LDS)
+    const uint32_t perCU = cfg.blocks_per_cu ? cfg.blocks_per_cu : residentBlocksFp32(cfg.low_energy_correction, waves, lds);
     const uint32_t regen = cfg.regeneration_threshold ? std::min<uint32_t>(cfg.regeneration_threshold, 64u) : 16u;
-    const uint32_t interact = cfg.interaction_threshold ? std::min<uint32_t>(cfg.interaction_threshold, 64u) : 24u;
+    const uint32_t interact = cfg.interaction_threshold ? std::min<uint32_t>(cfg.interaction_threshold, 64u) : 16u;
     // histories a wave claims per atomic on the global counter
     const uint32_t pool = cfg.queue_depth ? std::clamp<uint32_t>(cfg.queue_depth, 64u, 1u << 16) : 512u;
     uint64_t grid = static_cast<uint64_t>(c->numCU) * perCU;
     const uint64_t lanesNeeded = (he - hb + 63) / 64 * 64;
     grid = std::max<uint64_t>(1, std::min<uint64_t>(grid, (lanesNeeded + block - 1) / block));
-    hipLaunchKernelGGL(kernelFor(cfg.low_energy_correction, waves), dim3(static_cast<uint32_t>(grid)), dim3(block), 0, c->stream, w, t, exps,
-        expStart, nExp, hb, he, cfg.seed, cfg.first_history, tally, ctr, nx, regen, interact, pool);
+    hipLaunchKernelGGL(kernelFor(cfg.low_energy_correction, waves), dim3(static_cast<uint32_t>(grid)), dim3(block), lds, c->stream, w, t, exps,
+        expStart, nExp, hb, he, cfg.seed, cfg.first_history, tally, ctr, nx, regen, interact, pool, stage);
 }
 }

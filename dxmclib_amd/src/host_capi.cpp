@@ -1,5 +1,6 @@
 // Implementation of include/dxmc_host.h on top of the C++ host API.
 #include "dxmc_host.h"
+#include "host_internal.hpp"
 
 #include "dxmc/attenuationlut.hpp"
 #include "dxmc/devicetables.hpp"
@@ -25,12 +26,6 @@ namespace {
 std::mutex gMutex;
 std::string gError;
 
-int fail(int status, const std::string& msg)
-{
-    std::lock_guard lock(gMutex);
-    gError = msg;
-    return status;
-}
 
 bool allZero(const double* a, int n)
 {
@@ -90,6 +85,24 @@ void setupCT(S& src, const dxmc_source_spec& s)
         std::vector<double> a(s.bowtie_angles, s.bowtie_angles + s.bowtie_n), w(s.bowtie_weights, s.bowtie_weights + s.bowtie_n);
         src.setBowTieFilter(std::make_shared<BowTieFilter<double>>(a, w));
     }
+}
+
+std::array<double, 6> cosinesOrIdentity(const double* c)
+{
+    if (!c || allZero(c, 6))
+        return { 1, 0, 0, 0, 1, 0 };
+    return { c[0], c[1], c[2], c[3], c[4], c[5] };
+}
+
+}
+
+namespace dxmc_host_detail {
+
+int fail(int status, const std::string& msg)
+{
+    std::lock_guard lock(gMutex);
+    gError = msg;
+    return status;
 }
 
 std::unique_ptr<Source<double>> makeSource(const dxmc_source_spec& s)
@@ -181,13 +194,6 @@ std::unique_ptr<Source<double>> makeSource(const dxmc_source_spec& s)
     }
 }
 
-std::array<double, 6> cosinesOrIdentity(const double* c)
-{
-    if (!c || allZero(c, 6))
-        return { 1, 0, 0, 0, 1, 0 };
-    return { c[0], c[1], c[2], c[3], c[4], c[5] };
-}
-
 bool buildWorld(const dxmc_host_world& w, World<double>& world, std::string& why)
 {
     if (!w.material || !w.density || !w.material_names || w.n_materials == 0)
@@ -214,7 +220,12 @@ bool buildWorld(const dxmc_host_world& w, World<double>& world, std::string& why
         return why = "world failed validation", false;
     return true;
 }
+
 }
+
+using dxmc_host_detail::buildWorld;
+using dxmc_host_detail::fail;
+using dxmc_host_detail::makeSource;
 
 extern "C" {
 

@@ -191,7 +191,7 @@ typedef struct {
     uint32_t interaction_threshold; /* FP32: pending interactions per wave before they run, 0: tuned */
     uint32_t queue_depth; /* FP32: histories a wave claims per refill of its pool, 0: tuned */
     uint32_t waves_per_simd; /* FP32: kernel variant by register budget, 5 (0), 6 or 8 resident waves per SIMD */
-    uint32_t reserved; /* 0 */
+    uint32_t lds_tables; /* FP32: sampling tables in LDS, 1 none, 2 all but the attenuation coefficients, 3 all (0: tuned) */
 } dxmc_run_cfg;
 
 /* dxmc_run_cfg.flags */
@@ -219,8 +219,8 @@ int dxmc_hip_run(dxmc_ctx* ctx, const dxmc_world* world, const dxmc_tables* tabl
  * CPU threads need no tuning). Times trial launches of trial_histories
  * histories drawn evenly from the exposure table into a scratch tally and
  * searches, one parameter at a time from the defaults, over
- *   FP32: waves_per_simd, regeneration_threshold, interaction_threshold,
- *         queue_depth, blocks_per_cu;
+ *   FP32: waves_per_simd, lds_tables, regeneration_threshold,
+ *         interaction_threshold, queue_depth, blocks_per_cu;
  *   FP64: block_size.
  * Tuning fields that are non-zero in 'base' are kept fixed. 'best' receives
  * 'base' with the chosen values filled in; the choice is also kept in the

@@ -123,6 +123,71 @@ int dxmc_host_transport(const dxmc_host_world* world, const dxmc_source_spec* so
  * absorption [cm2/g] at energy E (keV) and the standard density. */
 int dxmc_host_material_data(const char* name, double energy, double out[5], double* density);
 
+/* ------------------------------------------------------------ serving
+ * A request server over one GPU (no counterpart in the reference, whose
+ * Transport runs one call at a time on the host cores). Each request is one
+ * Transport<T>::operator() call: a world and a source. Requests are
+ * prepared (attenuation tables, exposure table) by a pool of host threads,
+ * then admitted to the GPU in arrival order, as soon as one of 'slots'
+ * device slots is idle and the request's device memory fits in what the
+ * admitted requests leave of 'memory_budget'; a request that does not fit
+ * waits while later ones that fit go ahead. Each slot is a context with its
+ * own stream, so the kernels of admitted requests run concurrently. World,
+ * tables and tally stay on the device for the request's lifetime. Results
+ * depend only on the request (world, source, seed): each is the same alone
+ * and in any mix. */
+typedef struct dxmc_server dxmc_server;
+
+typedef struct {
+    int32_t device;
+    int32_t precision; /* dxmc_precision */
+    int32_t low_energy_correction; /* 0, 1, 2 */
+    uint32_t slots; /* requests on the device at once, 1..8 (0: 4) */
+    uint32_t prep_threads; /* host threads preparing requests (0: 4) */
+    uint32_t reserved;
+    uint64_t memory_budget; /* device bytes for admitted requests (0: 80 % of the free memory) */
+    dxmc_run_cfg tuning; /* precision, L and seed fields are ignored */
+} dxmc_server_cfg;
+
+enum dxmc_request_state {
+    DXMC_REQUEST_QUEUED = 0, /* submitted, being prepared */
+    DXMC_REQUEST_RUNNING = 1, /* admitted to a slot */
+    DXMC_REQUEST_DONE = 2, /* result ready (or failed: see status) */
+    DXMC_REQUEST_UNKNOWN = 3, /* no such request (never submitted or already collected) */
+    DXMC_REQUEST_READY = 4 /* prepared, waiting for a slot and memory */
+};
+
+typedef struct {
+    double submitted, prepared, admitted, finished; /* seconds on the server clock */
+    uint64_t histories, steps, events;
+    uint64_t device_bytes; /* memory charged against the budget */
+    int32_t status; /* DXMC_OK or the error of the request */
+    int32_t slot;
+} dxmc_request_info;
+
+int dxmc_host_server_create(const dxmc_server_cfg* cfg, dxmc_server** out);
+/* Stops accepting requests, finishes the admitted ones, drops the queued
+ * ones and frees everything. */
+int dxmc_host_server_destroy(dxmc_server* server);
+/* Submit a request; the world arrays and the source spec are copied before
+ * the call returns. output_mode: DXMC_OUTPUT_EV_PER_HISTORY, or
+ * DXMC_OUTPUT_DOSE in keV/kg (no source calibration). Fails with
+ * DXMC_ERR_ADMISSION if the request can never fit the memory budget. */
+int dxmc_host_server_submit(dxmc_server* server, const dxmc_host_world* world, const dxmc_source_spec* source, uint64_t seed,
+    int32_t output_mode, uint64_t* request_id);
+/* Wait up to timeout_seconds (< 0: forever) for the request; *state gets
+ * its dxmc_request_state. */
+int dxmc_host_server_wait(dxmc_server* server, uint64_t request_id, double timeout_seconds, int32_t* state);
+/* Copy a finished request's result (any array may be NULL) and info, and
+ * forget the request. */
+int dxmc_host_server_collect(dxmc_server* server, uint64_t request_id, double* dose, uint32_t* n_events, double* variance,
+    dxmc_request_info* info);
+/* Hold (paused != 0) or resume admission; requests keep being prepared
+ * while admission is held (draining a device, admitting a batch at once). */
+int dxmc_host_server_pause(dxmc_server* server, int32_t paused);
+/* Seconds on the server clock (the time base of dxmc_request_info). */
+double dxmc_host_server_clock(const dxmc_server* server);
+
 #ifdef __cplusplus
 }
 #endif
