@@ -48,6 +48,10 @@ def parse():
     p.add_argument("--waves-per-simd", type=int, default=0, help="FP32 kernel variant (5, 6, 8), 0: default")
     p.add_argument("--low-energy-correction", type=int, default=1)
     p.add_argument("--no-autotune", action="store_true", help="skip the autotuner (defaults or the fields given)")
+    p.add_argument("--no-serving", action="store_true", help="skip the request-stream (serving) measurement")
+    p.add_argument("--serving-requests", type=int, default=256)
+    p.add_argument("--serving-slots", type=int, default=4)
+    p.add_argument("--serving-prep-threads", type=int, default=min(16, os.cpu_count() or 4))
     return p.parse_args()
 
 
@@ -124,6 +128,50 @@ def cpu_baseline(wl, tables, exposures, ext, L, seconds):
     dt = time.perf_counter() - t0
     return {"value": r["histories"] / dt, "unit": "histories/s", "cores": threads, "kind": "port",
             "sample": f"{len(sample)} of {len(exposures)} exposures (every {stride}th) x {per} histories, FP64, L={L}, {dt:.1f} s"}
+
+
+def serving(n_requests, rate, slots, prep_threads):
+    """The request stream of the serving configuration: n_requests Poisson
+    arrivals (rate per second) of water cubes with a bone sphere (32-256
+    voxels a side), DX or CT axial at 80-140 kVp, 1e6-1e8 histories. Each is
+    submitted to the request server at its arrival time; latency is arrival
+    to result ready."""
+    import concurrent.futures as cf
+    from dxmclib_amd import backend, capi, workloads
+    reqs = workloads.serving_requests(n=n_requests, rate=rate)
+
+    def make(r):
+        mat, dens, names = workloads.cube_phantom(r["side"], r["spacing"], r["seed"])
+        return backend.host_world(mat, dens, (r["spacing"],) * 3, names), workloads.request_spec(r)
+
+    with backend.Server(slots=slots, prep_threads=prep_threads) as srv, cf.ThreadPoolExecutor(8) as pool:
+        made = [pool.submit(make, r) for r in reqs]
+        # worlds are generated ahead of their arrival times
+        for f in made[:min(16, len(made))]:
+            f.result()
+        t0 = time.perf_counter()
+        ids = []
+        for r, f in zip(reqs, made):
+            world, spec = f.result()
+            delay = r["arrival"] - (time.perf_counter() - t0)
+            if delay > 0:
+                time.sleep(delay)
+            ids.append(srv.submit(world, spec, seed=r["seed"]))
+        infos = []
+        for i in ids:
+            srv.wait(i)
+            infos.append(srv.collect(i)["info"])
+    lat = np.array([x.finished - x.submitted for x in infos])
+    wait = np.array([x.admitted - x.prepared for x in infos])
+    prep = np.array([x.prepared - x.submitted for x in infos])
+    run = np.array([x.finished - x.admitted for x in infos])
+    hist = sum(x.histories for x in infos)
+    span = max(x.finished for x in infos) - min(x.submitted for x in infos)
+    return {"requests": len(infos), "rate_per_s": rate, "slots": slots, "prep_threads": prep_threads,
+            "latency_p50_s": float(np.percentile(lat, 50)), "latency_p99_s": float(np.percentile(lat, 99)),
+            "latency_mean_s": float(lat.mean()), "prepare_mean_s": float(prep.mean()), "admission_wait_mean_s": float(wait.mean()),
+            "device_mean_s": float(run.mean()), "histories": int(hist), "makespan_s": span, "histories_per_s": hist / span,
+            "failed": int(sum(1 for x in infos if x.status != 0))}
 
 
 def spectrum_of(spec):
@@ -259,6 +307,9 @@ def main():
         dev.destroy_tables(t)
         dev.destroy_world(w)
 
+    if not args.no_serving:
+        result["serving"] = serving(args.serving_requests, 20.0, args.serving_slots, args.serving_prep_threads)
+        print(f"serving: {result['serving']}", file=sys.stderr, flush=True)
     if not args.no_cpu:
         result["cpu_baseline"] = cpu_baseline(wl, tables, exposures, ext, L, args.cpu_seconds)
         print(f"cpu: {result['cpu_baseline']['value']:.4e} histories/s", file=sys.stderr, flush=True)

@@ -87,6 +87,7 @@ struct Tables {
     const double* lutGrid; // log10 E of the nodes, both precisions
     const double* lutBound; // FP32: lowest energy of each segment on the FP64 path
     const float* lutAnchor; // FP32: log2 E of the nodes
+    float lutLinearEnergy2, lutLinearStepInv2; // FP32: linear index start and 1/step, log2 units
     const T* lutCoef; // [mat][node][type][b,a]
     const T* lutMajorant; // [node][b,a]
     const T* rita; // [mat][x,e,a,b][56]
@@ -264,9 +265,17 @@ __device__ inline LutPoint<double> lutPoint(const Tables<double>& t, double E)
 // without an FP64 logarithm.
 __device__ inline LutPoint<float> lutPoint(const Tables<float>& t, float E)
 {
-    constexpr float LOG10_2 = 0.30102999566398119521f;
     const float le2 = log2f(E);
-    uint32_t k = lutInterval(t, static_cast<double>(le2 * LOG10_2));
+    // first guess in float (the linear index, or a search of the anchors)
+    uint32_t k;
+    if (le2 > t.lutLinearEnergy2) {
+        const float f = (le2 - t.lutLinearEnergy2) * t.lutLinearStepInv2;
+        k = f < float(t.lutNodes) ? static_cast<uint32_t>(f) + t.lutLinearIndex : t.lutNodes - 1;
+        k = k < t.lutNodes - 1 ? k : t.lutNodes - 1;
+    } else {
+        k = upperBound(t.lutAnchor, t.lutNodes, le2);
+        k = k != t.lutNodes ? k : t.lutNodes - 1;
+    }
     const double e = E;
     while (k + 1 < t.lutNodes && e >= t.lutBound[k + 1])
         ++k;

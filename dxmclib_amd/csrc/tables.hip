@@ -143,6 +143,8 @@ namespace {
         out.lutLinearIndex = d.lut_linear_index;
         out.lutLinearStep = d.lut_linear_step;
         out.lutLinearEnergy = d.lut_linear_energy;
+        out.lutLinearEnergy2 = static_cast<float>(d.lut_linear_energy * LOG2_10);
+        out.lutLinearStepInv2 = static_cast<float>(1.0 / (d.lut_linear_step * LOG2_10));
         out.nSpectra = d.n_spectra;
         int st;
         if ((st = upload(c, t, cast<double>(d.lut_grid, d.lut_nodes), &out.lutGrid)))

@@ -104,10 +104,12 @@ __device__ inline const T* stageArray(const T* src, uint64_t count, unsigned cha
 }
 
 // copies the tables into LDS and points t at the copies (all threads of the
-// block call it; ends with a barrier)
-__device__ inline void stageTables(Tables<float>& t, uint32_t level, unsigned char* lds)
+// block call it; ends with a barrier). The level is a template parameter so
+// that the compiler sees which pointers address LDS.
+template <uint32_t level>
+__device__ inline void stageTables(Tables<float>& t, unsigned char* lds)
 {
-    if (level <= STAGE_NONE)
+    if constexpr (level <= STAGE_NONE)
         return;
     unsigned char* c = lds;
     t.rita = stageArray(t.rita, uint64_t(t.nMaterials) * 4 * 56, c);
@@ -126,7 +128,7 @@ __device__ inline void stageTables(Tables<float>& t, uint32_t level, unsigned ch
     t.bowtieOffset = stageArray(t.bowtieOffset, t.nBowtie + 1, c);
     t.bowtieAngle = stageArray(t.bowtieAngle, t.bowtieTotal, c);
     t.bowtieWeight = stageArray(t.bowtieWeight, t.bowtieTotal, c);
-    if (level >= STAGE_ALL)
+    if constexpr (level >= STAGE_ALL)
         t.lutCoef = stageArray(t.lutCoef, uint64_t(t.nMaterials) * t.lutNodes * 6, c);
     __syncthreads();
 }
@@ -143,16 +145,16 @@ __device__ inline bool roulette(Particle<float>& p, Rng& rng)
 }
 
 // W: minimum resident waves per SIMD the register allocation must allow
-// (1: unconstrained, about 5 with this kernel; 6 and 8 trade a few spills
-// for occupancy). The autotuner picks among the variants.
-template <int L, int W>
+// (1: unconstrained, about 5 with this kernel; 6 trades a few spills for
+// occupancy). S: LDS staging level. The autotuner picks among the variants.
+template <int L, int W, uint32_t S>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(W, 8))) void transportPersistent(World<float> w, Tables<float> tg, const Exposure<float>* exps, const uint64_t* expStart,
     uint32_t nExp, uint64_t hBegin, uint64_t hEnd, uint64_t seed, uint64_t firstHistory, Tally tally, Counters* ctr, unsigned long long* next,
-    uint32_t regenThreshold, uint32_t interactThreshold, uint32_t poolSize, uint32_t stage)
+    uint32_t regenThreshold, uint32_t interactThreshold, uint32_t poolSize)
 {
     extern __shared__ __align__(16) unsigned char lds[];
     Tables<float> t = tg;
-    stageTables(t, stage, lds);
+    stageTables<S>(t, lds);
     Particle<float> p;
     Rng rng { 0, 1 };
     float majInv = 0;
@@ -280,21 +282,23 @@ namespace dxmc_backend {
 namespace {
 
 using KernelFn = void (*)(World<float>, Tables<float>, const Exposure<float>*, const uint64_t*, uint32_t, uint64_t, uint64_t, uint64_t, uint64_t,
-    Tally, Counters*, unsigned long long*, uint32_t, uint32_t, uint32_t, uint32_t);
+    Tally, Counters*, unsigned long long*, uint32_t, uint32_t, uint32_t);
 
-template <int L>
+template <int L, uint32_t S>
 KernelFn variant(uint32_t waves)
 {
-    if (waves >= 8)
-        return transportPersistent<L, 8>;
-    if (waves >= 6)
-        return transportPersistent<L, 6>;
-    return transportPersistent<L, 1>;
+    return waves >= 6 ? transportPersistent<L, 6, S> : transportPersistent<L, 1, S>;
 }
 
-KernelFn kernelFor(int L, uint32_t waves)
+template <int L>
+KernelFn variant(uint32_t waves, uint32_t stage)
 {
-    return L == 0 ? variant<0>(waves) : L == 1 ? variant<1>(waves) : variant<2>(waves);
+    return stage >= STAGE_ALL ? variant<L, STAGE_ALL>(waves) : stage >= STAGE_SMALL ? variant<L, STAGE_SMALL>(waves) : variant<L, STAGE_NONE>(waves);
+}
+
+KernelFn kernelFor(int L, uint32_t waves, uint32_t stage)
+{
+    return L == 0 ? variant<0>(waves, stage) : L == 1 ? variant<1>(waves, stage) : variant<2>(waves, stage);
 }
 
 }
@@ -302,7 +306,8 @@ KernelFn kernelFor(int L, uint32_t waves)
 uint32_t residentBlocksFp32(int L, uint32_t waves, uint32_t ldsPerBlock)
 {
     int resident = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&resident, reinterpret_cast<const void*>(kernelFor(L, waves)), 256, ldsPerBlock) != hipSuccess
+    const uint32_t stage = ldsPerBlock ? STAGE_SMALL : STAGE_NONE;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&resident, reinterpret_cast<const void*>(kernelFor(L, waves, stage)), 256, ldsPerBlock) != hipSuccess
         || resident <= 0) {
         hipGetLastError();
         resident = 1;
@@ -351,7 +356,7 @@ void launchFp32(dxmc_ctx* c, const dxmc_run_cfg& cfg, const World<float>& w, con
     uint64_t grid = static_cast<uint64_t>(c->numCU) * perCU;
     const uint64_t lanesNeeded = (he - hb + 63) / 64 * 64;
     grid = std::max<uint64_t>(1, std::min<uint64_t>(grid, (lanesNeeded + block - 1) / block));
-    hipLaunchKernelGGL(kernelFor(cfg.low_energy_correction, waves), dim3(static_cast<uint32_t>(grid)), dim3(block), lds, c->stream, w, t, exps,
-        expStart, nExp, hb, he, cfg.seed, cfg.first_history, tally, ctr, nx, regen, interact, pool, stage);
+    hipLaunchKernelGGL(kernelFor(cfg.low_energy_correction, waves, stage), dim3(static_cast<uint32_t>(grid)), dim3(block), lds, c->stream, w, t,
+        exps, expStart, nExp, hb, he, cfg.seed, cfg.first_history, tally, ctr, nx, regen, interact, pool);
 }
 }

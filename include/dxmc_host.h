@@ -6,7 +6,7 @@
  * the sources (reference source.hpp getExposure/totalExposures) and runs the
  * complete Transport<T>::operator() (reference transport.hpp:135-202) on the
  * GPU backend of dxmc_hip.h. It exists for bindings that cannot use the C++
- * templates directly (ctypes, cgo, JNI); C++ users include dxmc/dxmc.hpp.
+ * templates directly (ctypes, cgo, JNI); C++ users include dxmc.hpp.
  */
 #ifndef DXMC_HOST_H
 #define DXMC_HOST_H
