@@ -30,11 +30,13 @@ HIP_FLAGS = [
     "-Wno-unused-result", "-Wno-unused-value",
 ]
 # per source: the FP64 plain path is compiled without FMA contraction so it
-# rounds like the CPU oracle; the FP32 path may contract.
+# rounds like the CPU oracle; the FP32 transport kernel may contract and uses
+# the hardware transcendental instructions (v_exp_f32, v_log_f32, v_sin_f32,
+# v_cos_f32) and reciprocal-based division and square root.
 HIP_SOURCES = {
     "dxmc_hip.hip": ["-ffp-contract=off"],
     "tables.hip": ["-ffp-contract=off"],
-    "transport_fp32.hip": [],
+    "transport_fp32.hip": ["-fgpu-approx-transcendentals", "-fno-hip-fp32-correctly-rounded-divide-sqrt"],
     "autotune.hip": [],
 }
 HOST_SOURCES = ["src/material.cpp", "src/host_capi.cpp", "src/server.cpp"]

@@ -349,8 +349,8 @@ void launchFp32(dxmc_ctx* c, const dxmc_run_cfg& cfg, const World<float>& w, con
     const uint32_t lds = ldsBytes(t, stage);
     // default: as many blocks as stay resident on a CU (registers and LDS)
     const uint32_t perCU = cfg.blocks_per_cu ? cfg.blocks_per_cu : residentBlocksFp32(cfg.low_energy_correction, waves, lds);
-    const uint32_t regen = cfg.regeneration_threshold ? std::min<uint32_t>(cfg.regeneration_threshold, 64u) : 16u;
-    const uint32_t interact = cfg.interaction_threshold ? std::min<uint32_t>(cfg.interaction_threshold, 64u) : 16u;
+    const uint32_t regen = cfg.regeneration_threshold ? std::min<uint32_t>(cfg.regeneration_threshold, 64u) : 20u;
+    const uint32_t interact = cfg.interaction_threshold ? std::min<uint32_t>(cfg.interaction_threshold, 64u) : 20u;
     // histories a wave claims per atomic on the global counter
     const uint32_t pool = cfg.queue_depth ? std::clamp<uint32_t>(cfg.queue_depth, 64u, 1u << 16) : 512u;
     uint64_t grid = static_cast<uint64_t>(c->numCU) * perCU;
