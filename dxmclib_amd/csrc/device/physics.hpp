@@ -265,7 +265,7 @@ __device__ inline LutPoint<double> lutPoint(const Tables<double>& t, double E)
 // without an FP64 logarithm.
 __device__ inline LutPoint<float> lutPoint(const Tables<float>& t, float E)
 {
-    const float le2 = log2f(E);
+    const float le2 = __builtin_amdgcn_logf(E); // log2 (v_log_f32); E >= 1 keV
     // first guess in float (the linear index, or a search of the anchors)
     uint32_t k;
     if (le2 > t.lutLinearEnergy2) {
@@ -284,7 +284,9 @@ __device__ inline LutPoint<float> lutPoint(const Tables<float>& t, float E)
     return { k, le2 - t.lutAnchor[k] };
 }
 __device__ inline double lutValue(const double* c, const LutPoint<double>& p) { return pow(10.0, c[0] + c[1] * p.x); }
-__device__ inline float lutValue(const float* c, const LutPoint<float>& p) { return exp2f(c[0] + c[1] * p.x); }
+// v_exp_f32 directly: the arguments (log2 of attenuations) are far from the
+// denormal range the library exp2f guards against
+__device__ inline float lutValue(const float* c, const LutPoint<float>& p) { return __builtin_amdgcn_exp2f(c[0] + c[1] * p.x); }
 
 // photo, Compton, Rayleigh mass attenuation [cm2/g] of a material at a LUT point
 template <typename T>

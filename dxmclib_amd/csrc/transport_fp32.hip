@@ -133,6 +133,15 @@ __device__ inline void stageTables(Tables<float>& t, unsigned char* lds)
     __syncthreads();
 }
 
+constexpr float LN2F = 0.693147180559945309f;
+
+// inside the safe extent, without short-circuit branches
+__device__ inline bool insideFast(const World<float>& w, const float* pos)
+{
+    return (pos[0] > w.extent[0]) & (pos[0] < w.extent[1]) & (pos[1] > w.extent[2]) & (pos[1] < w.extent[3]) & (pos[2] > w.extent[4])
+        & (pos[2] < w.extent[5]);
+}
+
 // Russian roulette after a step that left the photon alive (transport.hpp:684-694)
 __device__ inline bool roulette(Particle<float>& p, Rng& rng)
 {
@@ -214,7 +223,9 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(W, 8))) voi
                     rng = Rng::forHistory(seed, firstHistory + h);
                     sampleParticle(exps[e], t, rng, p);
                     phase = transportToWorld(w, p) ? FLYING : DEAD;
-                    update = true;
+                    lp = lutPoint(t, p.energy);
+                    majInv = majorantInverseAt(t, lp);
+                    update = false;
                     historySteps = 0;
                 }
                 poolNext += take;
@@ -238,37 +249,43 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(W, 8))) voi
                                        : computeInteractionsForced<float, L>(t, prob, att, p, mat, sc, idx, rng, update);
                 if (alive)
                     alive = roulette(p, rng);
+                if (alive && update) { // new energy: its LUT segment and majorant
+                    lp = lutPoint(t, p.energy);
+                    majInv = majorantInverseAt(t, lp);
+                    update = false;
+                }
                 phase = alive ? FLYING : DEAD;
             }
         } else if (phase == FLYING) {
-            if (update) {
-                lp = lutPoint(t, p.energy);
-                majInv = majorantInverseAt(t, lp);
-                update = false;
-            }
-            const float step = -logf(rng.uniform<float>()) * majInv * 10.0f; // cm -> mm
+            // one Woodcock flight (woodcockParticleTracking, transport.hpp:
+            // 650-699), written without branches apart from the rare
+            // roulette: the random draws the reference makes only inside
+            // the world (decision) are drawn and kept by selection
+            const float u1 = rng.uniform<float>();
+            const float step = -__builtin_amdgcn_logf(u1) * (LN2F * 10.0f) * majInv; // cm -> mm
             for (int i = 0; i < 3; ++i)
                 p.pos[i] += p.dir[i] * step;
-            if (!inside(w, p.pos)) {
-                phase = DEAD;
-            } else {
-                ++steps;
-                idx = voxelIndexFast(w, p.pos, inv);
-                readVoxel(w, idx, mat, density, meas);
-                float att[3];
-                attenuationAt(t, mat, lp, att);
-                const float prob = ((att[0] + att[1]) + att[2]) * density * majInv;
-                if (meas == 0) {
-                    if (rng.uniform<float>() < prob)
-                        phase = PENDING;
-                    else if (!roulette(p, rng))
-                        phase = DEAD;
-                } else {
-                    phase = PENDING;
-                }
-                if (phase != DEAD && ++historySteps >= MAX_STEPS)
-                    phase = DEAD;
+            const bool in = insideFast(w, p.pos);
+            idx = voxelIndexFast(w, p.pos, inv); // clamped: a valid voxel also outside
+            readVoxel(w, idx, mat, density, meas);
+            float att[3];
+            attenuationAt(t, mat, lp, att);
+            const float prob = ((att[0] + att[1]) + att[2]) * density * majInv;
+            Rng r2 = rng;
+            const float u2 = r2.uniform<float>();
+            const bool drawn = in & (meas == 0);
+            rng.s = drawn ? r2.s : rng.s;
+            const bool real = (meas != 0) | (u2 < prob);
+            uint32_t next = in ? (real ? PENDING : FLYING) : DEAD;
+            if (next == FLYING && p.energy * p.weight < 5.0f) { // Russian roulette
+                if (rng.uniform<float>() < 0.8f)
+                    next = DEAD;
+                else
+                    p.weight *= 1.0f / (1.0f - 0.8f);
             }
+            steps += in ? 1u : 0u;
+            historySteps += in ? 1u : 0u;
+            phase = (next != DEAD && historySteps >= MAX_STEPS) ? DEAD : next;
         }
     }
     if (ctr)
