@@ -109,7 +109,7 @@ int autotune(dxmc_ctx* c, const dxmc_world* w, const dxmc_tables* t, const dxmc_
             for (uint32_t b : { resident > 1 ? resident - 1 : 1u, std::max(1u, resident / 2) })
                 if (b != resident && std::find(perCU.begin(), perCU.end(), b) == perCU.end())
                     perCU.push_back(b);
-            if (!st && !(st = searchField(tr, cfg, &dxmc_run_cfg::regeneration_threshold, { 4, 8, 12, 16, 20, 24, 32, 40, 48 }, bestRate))
+            if (!st && !(st = searchField(tr, cfg, &dxmc_run_cfg::regeneration_threshold, { 1, 2, 8, 16 }, bestRate))
                 && !(st = searchField(tr, cfg, &dxmc_run_cfg::interaction_threshold, { 1, 8, 16, 24, 32, 40, 48 }, bestRate))
                 && !(st = searchField(tr, cfg, &dxmc_run_cfg::queue_depth, { 128, 256, 512, 1024, 2048, 4096 }, bestRate)))
                 st = searchField(tr, cfg, &dxmc_run_cfg::blocks_per_cu, perCU, bestRate);

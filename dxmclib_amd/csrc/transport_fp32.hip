@@ -15,12 +15,13 @@
 // So the long, branchy interaction samplers (Compton/Rayleigh rejection
 // loops, fluorescence) run with many lanes active instead of with the few
 // lanes that happen to interact in a given flight, while the cheap flights
-// run nearly full. Dead lanes are refilled when at least
-// 'regeneration_threshold' of them wait (or the whole wave is dead), from a
-// wave-private pool of history indices; one lane refills the pool with a
-// single atomic on the global counter (512 histories per claim, so the
-// counter is not a point of contention); each dead lane takes its rank among
-// the dead lanes (ballot + mbcnt).
+// run nearly full. New photons are sampled by the whole wave at once (64
+// histories from a wave-private pool of history indices, which one lane
+// refills with a single atomic on the global counter) into a per-wave queue
+// in LDS; dead lanes take photons from that queue once at least
+// 'regeneration_threshold' of them wait, each by its rank among the dead
+// lanes (ballot + mbcnt). So the costly source sampling runs with all lanes
+// active and a lane is refilled with a few LDS reads.
 //
 // Each history keeps its own PCG stream and consumes it in the reference's
 // order (flight, decision, interaction, roulette), so delaying an interaction
@@ -135,6 +136,13 @@ __device__ inline void stageTables(Tables<float>& t, unsigned char* lds)
 
 constexpr float LN2F = 0.693147180559945309f;
 
+// per-wave queue of sampled photons in LDS: position, direction, energy,
+// weight, PCG state and increment, LUT segment and offset, 1/majorant
+constexpr int QUEUE_FIELDS = 15;
+constexpr uint32_t QUEUE_CAP = 64;
+constexpr uint32_t QUEUE_BYTES = QUEUE_FIELDS * QUEUE_CAP * 4;
+constexpr uint32_t WAVES_PER_BLOCK = 4;
+
 // inside the safe extent, without short-circuit branches
 __device__ inline bool insideFast(const World<float>& w, const float* pos)
 {
@@ -186,23 +194,32 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(W, 8))) voi
     const float inv[3] = { 1.0f / w.spacing[0], 1.0f / w.spacing[1], 1.0f / w.spacing[2] };
     const uint32_t lane = __lane_id();
 
+    // the wave's queue of new photons, already in the world (LDS, SoA)
+    float* q = reinterpret_cast<float*>(lds + ldsBytes(tg, S) + (threadIdx.x / 64) * QUEUE_BYTES);
+    uint32_t queued = 0;
+
     for (;;) {
-        const uint64_t dead = __ballot(phase == DEAD);
-        const uint32_t nDead = __popcll(dead);
+        uint64_t dead = __ballot(phase == DEAD);
+        uint32_t nDead = __popcll(dead);
         const bool work = poolNext < poolEnd || !claimedAll;
-        if (work && nDead > 0 && (nDead >= regenThreshold || nDead == 64u)) {
-            // hand histories of the wave's pool to the dead lanes in rank
-            // order, refilling the pool from the global counter as needed
-            const uint32_t rank = rankBelow(dead);
+        if (work && queued < nDead) {
+            // sample new histories with the whole wave: lanes 0 .. want-1
+            // take the next histories of the wave's pool (refilled from the
+            // global counter as needed), photons that enter the world join
+            // the queue
+            const uint32_t want = QUEUE_CAP - queued;
             uint32_t assigned = 0;
-            while (assigned < nDead) {
+            bool sampled = false;
+            Particle<float> np;
+            Rng nr { 0, 1 };
+            while (assigned < want) {
                 if (poolNext >= poolEnd) {
                     if (claimedAll)
                         break;
                     unsigned long long base = 0;
-                    if (lane == __ffsll(static_cast<unsigned long long>(dead)) - 1)
+                    if (lane == 0)
                         base = atomicAdd(next, static_cast<unsigned long long>(poolSize));
-                    base = __shfl(base, __ffsll(static_cast<unsigned long long>(dead)) - 1);
+                    base = __shfl(base, 0);
                     if (base >= span) {
                         claimedAll = true;
                         break;
@@ -214,28 +231,64 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(W, 8))) voi
                     poolExp = upperBound(expStart, nExp + 1, poolNext) - 1;
                 }
                 const uint64_t avail = poolEnd - poolNext;
-                const uint32_t take = avail < nDead - assigned ? static_cast<uint32_t>(avail) : nDead - assigned;
-                if (phase == DEAD && rank >= assigned && rank < assigned + take) {
-                    const uint64_t h = poolNext + (rank - assigned);
+                const uint32_t take = avail < want - assigned ? static_cast<uint32_t>(avail) : want - assigned;
+                if (lane >= assigned && lane < assigned + take) {
+                    const uint64_t h = poolNext + (lane - assigned);
                     uint32_t e = poolExp;
                     while (h >= expStart[e + 1])
                         ++e;
-                    rng = Rng::forHistory(seed, firstHistory + h);
-                    sampleParticle(exps[e], t, rng, p);
-                    phase = transportToWorld(w, p) ? FLYING : DEAD;
-                    lp = lutPoint(t, p.energy);
-                    majInv = majorantInverseAt(t, lp);
-                    update = false;
-                    historySteps = 0;
+                    nr = Rng::forHistory(seed, firstHistory + h);
+                    sampleParticle(exps[e], t, nr, np);
+                    sampled = transportToWorld(w, np);
                 }
                 poolNext += take;
                 assigned += take;
             }
+            const uint64_t entered = __ballot(sampled);
+            if (sampled) {
+                const uint32_t slot = queued + rankBelow(entered);
+                const LutPoint<float> nl = lutPoint(t, np.energy);
+                const float nm = majorantInverseAt(t, nl);
+                const float v[QUEUE_FIELDS] = { np.pos[0], np.pos[1], np.pos[2], np.dir[0], np.dir[1], np.dir[2], np.energy, np.weight,
+                    __uint_as_float(static_cast<uint32_t>(nr.s)), __uint_as_float(static_cast<uint32_t>(nr.s >> 32)),
+                    __uint_as_float(static_cast<uint32_t>(nr.inc)), __uint_as_float(static_cast<uint32_t>(nr.inc >> 32)),
+                    __uint_as_float(nl.k), nl.x, nm };
+                for (int f = 0; f < QUEUE_FIELDS; ++f)
+                    q[f * QUEUE_CAP + slot] = v[f];
+            }
+            queued += __popcll(entered);
+        }
+        // dead lanes take photons from the queue (last in, first out)
+        if (nDead >= regenThreshold || (nDead > 0 && !work)) {
+            const uint32_t take = nDead < queued ? nDead : queued;
+            if (phase == DEAD) {
+                const uint32_t r = rankBelow(dead);
+                if (r < take) {
+                    const uint32_t slot = queued - 1 - r;
+                    float v[QUEUE_FIELDS];
+                    for (int f = 0; f < QUEUE_FIELDS; ++f)
+                        v[f] = q[f * QUEUE_CAP + slot];
+                    for (int i = 0; i < 3; ++i) {
+                        p.pos[i] = v[i];
+                        p.dir[i] = v[3 + i];
+                    }
+                    p.energy = v[6];
+                    p.weight = v[7];
+                    rng.s = static_cast<uint64_t>(__float_as_uint(v[9])) << 32 | __float_as_uint(v[8]);
+                    rng.inc = static_cast<uint64_t>(__float_as_uint(v[11])) << 32 | __float_as_uint(v[10]);
+                    lp = { __float_as_uint(v[12]), v[13] };
+                    majInv = v[14];
+                    update = false;
+                    historySteps = 0;
+                    phase = FLYING;
+                }
+            }
+            queued -= take;
         }
         const uint64_t pend = __ballot(phase == PENDING);
         const uint64_t fly = __ballot(phase == FLYING);
         if ((pend | fly) == 0) {
-            if (poolNext >= poolEnd && claimedAll)
+            if (poolNext >= poolEnd && claimedAll && queued == 0)
                 break;
             continue;
         }
@@ -339,7 +392,8 @@ uint32_t stageLevelFp32(const dxmc_run_cfg& cfg, const Tables<float>& t)
 {
     // a block gets at most 64 KiB of dynamic LDS
     constexpr uint32_t ldsPerBlock = 64 * 1024;
-    auto fits = [&](uint32_t level) { return ldsBytes(t, level) <= ldsPerBlock; };
+    auto bytes = [&](uint32_t level) { return uint64_t(ldsBytes(t, level)) + WAVES_PER_BLOCK * QUEUE_BYTES; };
+    auto fits = [&](uint32_t level) { return bytes(level) <= ldsPerBlock; };
     if (cfg.lds_tables >= STAGE_NONE && cfg.lds_tables <= STAGE_ALL) {
         uint32_t level = cfg.lds_tables;
         while (level > STAGE_NONE && !fits(level))
@@ -348,9 +402,9 @@ uint32_t stageLevelFp32(const dxmc_run_cfg& cfg, const Tables<float>& t)
     }
     const uint32_t blocks = residentBlocksFp32(cfg.low_energy_correction, cfg.waves_per_simd, 0);
     constexpr uint32_t ldsPerCU = 160 * 1024;
-    if (fits(STAGE_ALL) && uint64_t(ldsBytes(t, STAGE_ALL)) * blocks <= ldsPerCU)
+    if (fits(STAGE_ALL) && bytes(STAGE_ALL) * blocks <= ldsPerCU)
         return STAGE_ALL;
-    if (fits(STAGE_SMALL) && uint64_t(ldsBytes(t, STAGE_SMALL)) * blocks <= ldsPerCU)
+    if (fits(STAGE_SMALL) && bytes(STAGE_SMALL) * blocks <= ldsPerCU)
         return STAGE_SMALL;
     return STAGE_NONE;
 }
@@ -363,10 +417,10 @@ void launchFp32(dxmc_ctx* c, const dxmc_run_cfg& cfg, const World<float>& w, con
     const uint32_t block = 256;
     const uint32_t waves = cfg.waves_per_simd;
     const uint32_t stage = stageLevelFp32(cfg, t);
-    const uint32_t lds = ldsBytes(t, stage);
+    const uint32_t lds = ldsBytes(t, stage) + WAVES_PER_BLOCK * QUEUE_BYTES;
     // default: as many blocks as stay resident on a CU (registers and LDS)
     const uint32_t perCU = cfg.blocks_per_cu ? cfg.blocks_per_cu : residentBlocksFp32(cfg.low_energy_correction, waves, lds);
-    const uint32_t regen = cfg.regeneration_threshold ? std::min<uint32_t>(cfg.regeneration_threshold, 64u) : 20u;
+    const uint32_t regen = cfg.regeneration_threshold ? std::min<uint32_t>(cfg.regeneration_threshold, 64u) : 4u;
     const uint32_t interact = cfg.interaction_threshold ? std::min<uint32_t>(cfg.interaction_threshold, 64u) : 20u;
     // histories a wave claims per atomic on the global counter
     const uint32_t pool = cfg.queue_depth ? std::clamp<uint32_t>(cfg.queue_depth, 64u, 1u << 16) : 512u;

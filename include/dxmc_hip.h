@@ -186,7 +186,7 @@ typedef struct {
     uint64_t histories_per_launch; /* 0: automatic; bounds cancel/progress latency */
     uint32_t block_size; /* 0: tuned default */
     uint32_t blocks_per_cu; /* FP32 persistent kernel occupancy, 0: tuned default */
-    uint32_t regeneration_threshold; /* FP32: dead lanes per wave before refill, 0: tuned */
+    uint32_t regeneration_threshold; /* FP32: dead lanes per wave before they take queued new photons, 0: tuned */
     uint32_t flags; /* DXMC_RUN_* bits */
     uint32_t interaction_threshold; /* FP32: pending interactions per wave before they run, 0: tuned */
     uint32_t queue_depth; /* FP32: histories a wave claims per refill of its pool, 0: tuned */
