@@ -45,7 +45,8 @@ def parse():
     p.add_argument("--threshold", type=int, default=0, help="FP32 regeneration threshold (dead lanes per wave), 0: default")
     p.add_argument("--interaction-threshold", type=int, default=0, help="FP32 pending interactions per wave before they run, 0: default")
     p.add_argument("--queue-depth", type=int, default=0, help="FP32 histories per pool refill, 0: default")
-    p.add_argument("--waves-per-simd", type=int, default=0, help="FP32 kernel variant (5, 6, 8), 0: default")
+    p.add_argument("--waves-per-simd", type=int, default=0, help="FP32 kernel variant (5, 6), 0: default")
+    p.add_argument("--lds-tables", type=int, default=0, help="FP32 tables in LDS: 1 none, 2 all but the LUT, 3 all; 0: default")
     p.add_argument("--low-energy-correction", type=int, default=1)
     p.add_argument("--no-autotune", action="store_true", help="skip the autotuner (defaults or the fields given)")
     p.add_argument("--no-serving", action="store_true", help="skip the request-stream (serving) measurement")
@@ -174,6 +175,21 @@ def serving(n_requests, rate, slots, prep_threads):
             "failed": int(sum(1 for x in infos if x.status != 0))}
 
 
+def measured_traffic(workload, histories_per_launch):
+    """HBM bytes per launch (GB) of the transport kernel from the committed
+    rocprofv3 FETCH_SIZE / WRITE_SIZE passes of this workload (scaled by the
+    copy-kernel calibration), per history times this run's launch size; None
+    when no profile of the workload is committed."""
+    path = os.path.join(ROOT, "profiles", "transport_fp32_ct_spiral.json")
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        prof = json.load(f)
+    if prof.get("workload") != workload:
+        return None
+    return (prof["fetch_bytes_per_history"] + prof["write_bytes_per_history"]) * histories_per_launch / 1e9
+
+
 def spectrum_of(spec):
     import ctypes as C
     from dxmclib_amd import capi
@@ -212,6 +228,7 @@ def main():
     tuning.interaction_threshold = args.interaction_threshold
     tuning.queue_depth = args.queue_depth
     tuning.waves_per_simd = args.waves_per_simd
+    tuning.lds_tables = args.lds_tables
     tuning.flags = 1
 
     with backend.Device(0) as dev:
@@ -225,7 +242,7 @@ def main():
             tune_report = {"seconds": time.perf_counter() - t1, "trials": rep.trials, "default_rate": rep.default_rate,
                            "best_rate": rep.best_rate, "regeneration_threshold": tuning.regeneration_threshold,
                            "interaction_threshold": tuning.interaction_threshold, "queue_depth": tuning.queue_depth,
-                           "blocks_per_cu": tuning.blocks_per_cu, "waves_per_simd": tuning.waves_per_simd}
+                           "blocks_per_cu": tuning.blocks_per_cu, "waves_per_simd": tuning.waves_per_simd, "lds_tables": tuning.lds_tables}
             print(f"autotune: {tune_report}", file=sys.stderr, flush=True)
         tally = dev.tally(n)
         for i in range(args.warmup):
@@ -279,6 +296,7 @@ def main():
         }
         result["roofline"]["achieved"] = achieved / 1e9
         result["roofline"]["peak"] = HBM_PEAK / 1e9
+        result["roofline"]["traffic"] = measured_traffic(wl.name, histories / max(launches, 1))
         if not args.no_fp64:
             sub = subset(exposures, args.fp64_stride)
             sub_hist = sum(x.n_histories for x in sub)

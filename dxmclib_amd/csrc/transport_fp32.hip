@@ -386,8 +386,7 @@ uint32_t residentBlocksFp32(int L, uint32_t waves, uint32_t ldsPerBlock)
 }
 
 // staging level of a run: the requested one, or by default all tables when
-// they fit next to the register-limited number of resident blocks, else all
-// but the attenuation coefficients
+// three blocks of them fit a CU, else all but the attenuation coefficients
 uint32_t stageLevelFp32(const dxmc_run_cfg& cfg, const Tables<float>& t)
 {
     // a block gets at most 64 KiB of dynamic LDS
@@ -400,11 +399,14 @@ uint32_t stageLevelFp32(const dxmc_run_cfg& cfg, const Tables<float>& t)
             --level;
         return level;
     }
-    const uint32_t blocks = residentBlocksFp32(cfg.low_energy_correction, cfg.waves_per_simd, 0);
+    // all tables while at least 3 blocks (12 waves) stay resident per CU:
+    // the attenuation coefficients in LDS are worth more than the 4th and
+    // 5th block (measured on the CT workload)
+    const uint32_t blocks = std::min<uint32_t>(3, residentBlocksFp32(cfg.low_energy_correction, cfg.waves_per_simd, 0));
     constexpr uint32_t ldsPerCU = 160 * 1024;
     if (fits(STAGE_ALL) && bytes(STAGE_ALL) * blocks <= ldsPerCU)
         return STAGE_ALL;
-    if (fits(STAGE_SMALL) && bytes(STAGE_SMALL) * blocks <= ldsPerCU)
+    if (fits(STAGE_SMALL))
         return STAGE_SMALL;
     return STAGE_NONE;
 }
