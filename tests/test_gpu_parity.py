@@ -306,3 +306,132 @@ def test_autotune_keeps_results(device):
     finally:
         device.destroy_tables(tt)
         device.destroy_world(w)
+
+
+def _oracle_run(wl, tables, exps, maxd, ext, L, max_e, seed=42, measurement=None):
+    """The oracle on the exposures and source tables the device ran."""
+    orc = oracle.Oracle(wl.names, maxd, max_e, 1.0)
+    spectra = [_spectrum(wl.spec)] if wl.spec.type not in (capi.SOURCE_PENCIL,) else []
+    orc.add_source_tables(tables.source_arrays(), spectra)
+    ow = oracle.world_view(wl.material, wl.density.astype(np.float64), wl.spacing, ext, measurement)
+    return orc.transport(ow, exps, L=L, seed=seed)
+
+
+@pytest.mark.parametrize("L", [1, 2])
+def test_transport_dx_fp64(device, L):
+    """DX projection (120 kVp spectrum, heel) through the thorax phantom at
+    reduced histories: the FP64 path follows the oracle's histories."""
+    print(f"transport DX L={L}", flush=True)
+    wl = workloads.dx_thorax(histories_per_exposure=5000, exposures=20)
+    dose, ev, var, stats, t, exps, ext, maxd = _run_device(device, wl, L, capi.PRECISION_FP64)
+    ref = _oracle_run(wl, t, exps, maxd, ext, L, 120.0)
+    rel = abs(dose.sum() - ref["dose"].sum()) / ref["dose"].sum()
+    frac, n = _zstats(dose, var, ref["dose"], ref["variance"])
+    print(f"  total rel {rel:.2e}, |z|<3 {frac:.5f} of {n}, steps {stats.steps} vs {ref['steps']}, events {stats.events} vs {ref['events']}")
+    # identical streams; device and host math libraries may differ in the
+    # last bit of a transcendental, which lets a rare history take another path
+    assert rel < 5e-3
+    assert frac >= 0.997
+    assert abs(stats.steps - ref["steps"]) <= 1e-3 * ref["steps"]
+    assert abs(stats.events - ref["events"]) <= 1e-3 * ref["events"]
+    # the FP32 path follows the same histories
+    d32, _, v32, s32, *_ = _run_device(device, wl, L, capi.PRECISION_FP32)
+    rel32 = abs(d32.sum() - dose.sum()) / dose.sum()
+    print(f"  FP32: total rel {rel32:.2e}, steps {s32.steps}, events {s32.events}")
+    assert rel32 < 1e-3
+    assert abs(s32.steps - stats.steps) <= 1e-3 * stats.steps
+
+
+def test_transport_ct_spiral_fp64(device):
+    """CT spiral with bowtie and heel through the 512x512x300 phantom, 36 of
+    the 3600 exposures at 2000 histories: FP64 path against the oracle."""
+    wl = workloads.ct_body()
+    ext = backend.world_extent(wl.material.shape[::-1], wl.spacing)
+    maxd = [float(wl.density[wl.material == i].max()) if np.any(wl.material == i) else 0.0 for i in range(len(wl.names))]
+    t = backend.HostTables(wl.names, maxd, 120.0, 1.0)
+    allx = t.exposures(wl.spec)
+    sub = (capi.Exposure * 36)()
+    for j in range(36):
+        sub[j] = allx[j * 100]
+        sub[j].n_histories = 2000
+    w = device.upload_world(wl.material, wl.density, wl.spacing, ext, len(wl.names))
+    tt = device.upload_tables(t.desc())
+    out = {}
+    try:
+        for prec in (capi.PRECISION_FP64, capi.PRECISION_FP32):
+            tally = device.tally(wl.voxels)
+            stats = device.run(w, tt, sub, tally, prec, 1, seed=42)
+            d, ev, v = device.download(tally, wl.voxels)
+            device.destroy_tally(tally)
+            out[prec] = (d, v, stats)
+    finally:
+        device.destroy_tables(tt)
+        device.destroy_world(w)
+    orc = oracle.Oracle(wl.names, maxd, 120.0, 1.0)
+    orc.add_source_tables(t.source_arrays(), [_spectrum(wl.spec)])
+    ow = oracle.world_view(wl.material, wl.density.astype(np.float64), wl.spacing, ext)
+    ref = orc.transport(ow, sub, L=1, seed=42)
+    d, v, s = out[capi.PRECISION_FP64]
+    rel = abs(d.sum() - ref["dose"].sum()) / ref["dose"].sum()
+    frac, n = _zstats(d, v, ref["dose"], ref["variance"])
+    print(f"CT FP64: total rel {rel:.2e}, |z|<3 {frac:.5f} of {n}, steps {s.steps} vs {ref['steps']}, events {s.events} vs {ref['events']}",
+          flush=True)
+    assert rel < 5e-3
+    assert frac >= 0.997
+    assert abs(s.steps - ref["steps"]) <= 1e-3 * ref["steps"]
+    assert abs(s.events - ref["events"]) <= 1e-3 * ref["events"]
+    d32, v32, s32 = out[capi.PRECISION_FP32]
+    rel32 = abs(d32.sum() - d.sum()) / d.sum()
+    print(f"CT FP32: total rel {rel32:.2e}, steps {s32.steps}, events {s32.events}", flush=True)
+    assert rel32 < 1e-3
+    assert abs(s32.steps - s.steps) <= 1e-3 * s.steps
+
+
+def test_transport_forced_fp64(device):
+    """Forced interactions in flagged voxels (computeInteractionsForced,
+    transport.hpp:523-581): a water cylinder with a measurement region,
+    isotropic-CT source; FP64 against the oracle, FP32 against FP64."""
+    shape = (20, 48, 48)
+    mat = np.zeros(shape, dtype=np.uint8)
+    z, y, x = np.meshgrid(np.arange(20), np.arange(48), np.arange(48), indexing="ij")
+    mat[(x - 23.5) ** 2 + (y - 23.5) ** 2 < 20 ** 2] = 1
+    dens = np.array([0.001205, 1.0], dtype=np.float32)[mat]
+    meas = np.zeros(shape, dtype=np.uint8)
+    meas[6:14, 20:28, 20:28] = 1
+    names = ["Air, Dry (near sea level)", "Water, Liquid"]
+    # pencil along +y through the flagged block (cosines (z, x): beam = z cross x = +y)
+    spec = backend.source_spec(capi.SOURCE_PENCIL, position=(0.5, -100.0, 0.5), direction_cosines=(0, 0, 1, 1, 0, 0), photon_energy=70.0,
+                               histories_per_exposure=40000, total_exposures=5)
+    wl = workloads.Workload("forced", mat, dens, (1.0, 1.0, 2.0), names, spec, 200000)
+    ext = backend.world_extent(shape[::-1], wl.spacing)
+    maxd = [0.001205, 1.0]
+    t = backend.HostTables(names, maxd, 70.0, 1.0)
+    exps = t.exposures(spec)
+    w = device.upload_world(mat, dens, wl.spacing, ext, 2, measurement=meas)
+    tt = device.upload_tables(t.desc())
+    out = {}
+    try:
+        for prec in (capi.PRECISION_FP64, capi.PRECISION_FP32):
+            tally = device.tally(mat.size)
+            stats = device.run(w, tt, exps, tally, prec, 1, seed=42)
+            d, ev, v = device.download(tally, mat.size)
+            device.destroy_tally(tally)
+            out[prec] = (d, v, stats)
+    finally:
+        device.destroy_tables(tt)
+        device.destroy_world(w)
+    orc = oracle.Oracle(names, maxd, 70.0, 1.0)
+    ow = oracle.world_view(mat, dens.astype(np.float64), wl.spacing, ext, meas)
+    ref = orc.transport(ow, exps, L=1, seed=42)
+    d, v, s = out[capi.PRECISION_FP64]
+    flagged = meas.ravel() == 1
+    rel = abs(d[flagged].sum() - ref["dose"][flagged].sum()) / ref["dose"][flagged].sum()
+    print(f"forced FP64: flagged rel {rel:.2e}, steps {s.steps} vs {ref['steps']}, events {s.events} vs {ref['events']}", flush=True)
+    assert d[flagged].sum() > 0
+    assert rel < 5e-3
+    assert abs(s.steps - ref["steps"]) <= 1e-3 * ref["steps"]
+    assert abs(s.events - ref["events"]) <= 1e-3 * ref["events"]
+    d32, v32, s32 = out[capi.PRECISION_FP32]
+    rel32 = abs(d32[flagged].sum() - d[flagged].sum()) / d[flagged].sum()
+    print(f"forced FP32: flagged rel {rel32:.2e}", flush=True)
+    assert rel32 < 1e-3
