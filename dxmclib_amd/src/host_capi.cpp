@@ -374,6 +374,39 @@ int dxmc_host_transport(const dxmc_host_world* w, const dxmc_source_spec* spec, 
     return DXMC_OK;
 }
 
+int dxmc_host_ctdi_phantom(uint64_t diameter, uint64_t dims[3], double spacing[3], uint8_t* material, double* density, uint8_t* measurement,
+    uint8_t* hole, uint64_t capacity)
+{
+    if (!dims || !spacing || diameter == 0 || diameter > 4096)
+        return fail(DXMC_ERR_ARGUMENT, "ctdi_phantom: bad arguments");
+    const CTDIPhantom<double> w(diameter);
+    const auto& d = w.dimensions();
+    const auto& sp = w.spacing();
+    for (int i = 0; i < 3; ++i) {
+        dims[i] = d[i];
+        spacing[i] = sp[i];
+    }
+    if (!material && !density && !measurement && !hole)
+        return DXMC_OK;
+    if (capacity < w.size())
+        return fail(DXMC_ERR_ARGUMENT, "ctdi_phantom: capacity below the voxel count");
+    if (material)
+        std::copy(w.materialIndexArray()->cbegin(), w.materialIndexArray()->cend(), material);
+    if (density)
+        std::copy(w.densityArray()->cbegin(), w.densityArray()->cend(), density);
+    if (measurement)
+        std::copy(w.measurementMapArray()->cbegin(), w.measurementMapArray()->cend(), measurement);
+    if (hole) {
+        std::fill(hole, hole + w.size(), 0);
+        using H = CTDIPhantom<double>::HolePosition;
+        const H order[5] = { H::Center, H::West, H::East, H::South, H::North };
+        for (int i = 0; i < 5; ++i)
+            for (const auto k : w.holeIndices(order[i]))
+                hole[k] = static_cast<uint8_t>(i + 1);
+    }
+    return DXMC_OK;
+}
+
 int dxmc_host_material_data(const char* name, double E, double out[5], double* density)
 {
     if (!name || !out)

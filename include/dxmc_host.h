@@ -123,6 +123,15 @@ int dxmc_host_transport(const dxmc_host_world* world, const dxmc_source_spec* so
  * absorption [cm2/g] at energy E (keV) and the standard density. */
 int dxmc_host_material_data(const char* name, double energy, double out[5], double* density);
 
+/* The CTDI phantom the CT sources calibrate on (CTDIPhantom, reference
+ * world.hpp:229-390): PMMA cylinder of 'diameter_mm' with five air holes,
+ * 1 x 1 x 2.5 mm voxels, materials 0 air, 1 air (holes), 2 PMMA. hole[i] is
+ * the measured hole of voxel i (1 centre, 2 west, 3 east, 4 south, 5 north,
+ * 0 none); the measurement map flags the same voxels. With the arrays NULL
+ * only dims and spacing are returned. */
+int dxmc_host_ctdi_phantom(uint64_t diameter_mm, uint64_t dims[3], double spacing[3], uint8_t* material, double* density, uint8_t* measurement,
+    uint8_t* hole, uint64_t capacity);
+
 /* ------------------------------------------------------------ serving
  * A request server over one GPU (no counterpart in the reference, whose
  * Transport runs one call at a time on the host cores). Each request is one

@@ -1,0 +1,107 @@
+"""CT dose calibration (CTBaseSource::ctCalibration, reference
+source.hpp:925-988, on CTDIPhantom, world.hpp:229-390): the nested transport
+of a CT source through the CTDI phantom with forced interactions in the
+measurement holes, and CTDIvol / CTDIw / mean beam weight.
+
+CPU: the phantom geometry (disc, holes, measurement map).
+GPU: the calibration factor the host API applies in DOSE mode equals the
+factor restated here from the same phantom transported through the device C
+ABI (identical streams, FP64), and the centre/periphery ratio is physical.
+"""
+import ctypes as C
+import math
+
+import numpy as np
+import pytest
+
+from dxmclib_amd import backend, capi, workloads
+
+
+def _phantom(diameter):
+    dims = (C.c_uint64 * 3)()
+    sp = np.zeros(3)
+    lib = capi.host()
+    assert lib.dxmc_host_ctdi_phantom(diameter, dims, sp.ctypes.data_as(C.POINTER(C.c_double)), None, None, None, None, 0) == 0
+    n = dims[0] * dims[1] * dims[2]
+    mat = np.zeros(n, dtype=np.uint8)
+    dens = np.zeros(n)
+    meas = np.zeros(n, dtype=np.uint8)
+    hole = np.zeros(n, dtype=np.uint8)
+    u8 = C.POINTER(C.c_uint8)
+    assert lib.dxmc_host_ctdi_phantom(diameter, dims, sp.ctypes.data_as(C.POINTER(C.c_double)), mat.ctypes.data_as(u8),
+                                      dens.ctypes.data_as(C.POINTER(C.c_double)), meas.ctypes.data_as(u8), hole.ctypes.data_as(u8), n) == 0
+    shape = (dims[2], dims[1], dims[0])
+    return mat.reshape(shape), dens.reshape(shape), meas.reshape(shape), hole.reshape(shape), tuple(sp)
+
+
+def test_ctdi_phantom_geometry():
+    mat, dens, meas, hole, sp = _phantom(320)
+    assert mat.shape == (60, 323, 323) and sp == (1.0, 1.0, 2.5)
+    # PMMA disc of radius 160 mm: area within 1 % of pi r^2 (holes are air)
+    slice_ = mat[30]
+    pmma_or_hole = np.count_nonzero(slice_ >= 1)
+    assert abs(pmma_or_hole - math.pi * 160 ** 2) / (math.pi * 160 ** 2) < 0.01
+    # five holes of radius 6.55 mm, air, measured on |z| <= 50 mm (40 slices)
+    for h in range(1, 6):
+        vox = hole == h
+        per_slice = np.count_nonzero(vox[30])
+        assert abs(per_slice - math.pi * 6.55 ** 2) < 0.2 * math.pi * 6.55 ** 2
+        assert np.count_nonzero(vox.any(axis=(1, 2))) == 40
+        assert np.all(mat[vox] == 1) and np.allclose(dens[vox], 0.001205)
+    assert np.array_equal(meas > 0, hole > 0)
+
+
+def _ct_spec(histories_per_exposure, step_deg, ctdi_vol, **kw):
+    ang, w = workloads.bowtie_profile()
+    return backend.source_spec(capi.SOURCE_CT_AXIAL, tube_voltage=120.0, al_mm=6.0, sdd=1190.0, fov=500.0, collimation=38.4,
+                               exposure_angle_step_deg=step_deg, histories_per_exposure=histories_per_exposure, bowtie_angles=ang,
+                               bowtie_weights=w, heel_effect=1, ctdi_vol=ctdi_vol, **kw)
+
+
+@pytest.mark.gpu
+def test_ct_calibration_factor(device):
+    per, step, ctdivol = 100_000, 10.0, 10.0
+    spec = _ct_spec(per, step, ctdivol)
+    # a small target world: the factor does not depend on it
+    tw = backend.host_world(np.ones((8, 16, 16), dtype=np.uint8), np.ones((8, 16, 16), dtype=np.float32), (2.0, 2.0, 2.0),
+                            ["Air, Dry (near sea level)", "Water, Liquid"])
+    cal = backend.transport(tw, spec, precision=capi.PRECISION_FP64, output_mode=capi.OUTPUT_DOSE, use_source_calibration=True)
+    raw = backend.transport(tw, spec, precision=capi.PRECISION_FP64, output_mode=capi.OUTPUT_DOSE, use_source_calibration=False)
+    m = raw["dose"] > 0
+    factors = cal["dose"][m] / raw["dose"][m]
+    factor = float(np.median(factors))
+    assert np.allclose(factors, factor, rtol=1e-12)
+
+    # the same calibration restated: mean beam weight of the source, then
+    # the source moved to the phantom centre (cosines (-1,0,0,0,0,1)), one
+    # rotation, histories raised to at least 1e8 in total, FP64, seed 42
+    names = ["Air, Dry (near sea level)", "Air, Dry (near sea level)", "Polymethyl Methacralate (Lucite, Perspex)"]
+    t0 = backend.HostTables(names, [0.001205, 0.001205, 1.19], 120.0, 1.0)
+    mean_weight = float(np.mean([e.weight for e in t0.exposures(spec)]))
+    per_rotation = int(2 * math.pi / (step * math.pi / 180.0))
+    stat = max(1, 100_000_000 // (per_rotation * per))
+    cspec = _ct_spec(per * stat, step, ctdivol, direction_cosines=(-1, 0, 0, 0, 0, 1), position=(0.0, 0.0, 0.0), scan_length=38.4)
+    mat, dens, meas, hole, sp = _phantom(320)
+    ext = backend.world_extent(mat.shape[::-1], sp)
+    maxd = [float(dens[mat == i].max()) for i in range(3)]
+    t = backend.HostTables(names, maxd, 120.0, 1.0)
+    exps = t.exposures(cspec)
+    assert len(exps) == per_rotation
+    w = device.upload_world(mat, dens, sp, ext, 3, measurement=meas)
+    tt = device.upload_tables(t.desc())
+    tally = device.tally(mat.size)
+    try:
+        stats = device.run(w, tt, exps, tally, capi.PRECISION_FP64, 1, seed=42)
+        device.normalize(tally, w, capi.OUTPUT_DOSE, stats.histories, 1.0)
+        d, _, _ = device.download(tally, mat.size)
+    finally:
+        device.destroy_tally(tally)
+        device.destroy_tables(tt)
+        device.destroy_world(w)
+    flat = hole.ravel()
+    means = [d[flat == h].mean() for h in range(1, 6)]
+    ctdiw = (means[0] + 2 * np.mean(means[1:])) / 3 * 100.0 / 38.4
+    expected = ctdivol / ctdiw / mean_weight
+    print(f"calibration factor {factor:.6e}, restated {expected:.6e}; centre/periphery {means[0] / np.mean(means[1:]):.3f}", flush=True)
+    assert abs(factor - expected) <= 1e-9 * expected
+    assert 0.25 < means[0] / np.mean(means[1:]) < 0.9
