@@ -7,6 +7,10 @@
 // The host builds them; Transport flattens them into dxmc_tables_desc for the
 // device (csrc/) and the kernels sample exactly these values.
 #pragma once
+#include <string>
+#include <tuple>
+#include <mutex>
+#include <map>
 
 #include "dxmc/attenuationinterpolator.hpp"
 #include "dxmc/constants.hpp"
@@ -96,6 +100,26 @@ protected:
     // RITA over q^2 in [0, q_app], where q_app is grown (as a momentum
     // transfer: +0.5 while F^2 > 0.5, else +0.1) until F^2(q_app) <= 0.001 or
     // q_app reaches qmax^2 (reference attenuationlut.hpp:212-233)
+    // A built table depends only on the material's composition and its range,
+    // so the tables are kept per (composition, range) for the life of the
+    // process: a server that sees the same materials again skips the build.
+    template <typename Table, typename Build>
+    static Table cachedTable(int kind, const Material& m, T range, Build build)
+    {
+        static std::mutex mutex;
+        static std::map<std::tuple<int, std::string, T>, Table> cache;
+        const auto key = std::make_tuple(kind, m.compositionKey(), range);
+        {
+            std::lock_guard lock(mutex);
+            auto it = cache.find(key);
+            if (it != cache.end())
+                return it->second;
+        }
+        Table t = build();
+        std::lock_guard lock(mutex);
+        return cache.emplace(key, std::move(t)).first->second;
+    }
+
     void generateFormFactors(const std::vector<Material>& materials)
     {
         m_formFactor.clear();
@@ -108,7 +132,9 @@ protected:
                 qApp += ff > T { 0.5 } ? T { 0.5 } : T { 0.1 };
                 ff = m.getRayleightFormFactorSquared(qApp);
             }
-            m_formFactor.emplace_back(T { 0 }, qApp, [&](T qsqr) -> T { return m.getRayleightFormFactorSquared(std::sqrt(qsqr)); });
+            m_formFactor.push_back(cachedTable<RITA<T, RITA_NODES>>(0, m, qApp, [&] {
+                return RITA<T, RITA_NODES>(T { 0 }, qApp, [&](T qsqr) -> T { return m.getRayleightFormFactorSquared(std::sqrt(qsqr)); });
+            }));
         }
     }
     // spline of S(q)/Z over [0, q_app]; q_app grows after each evaluation until
@@ -124,7 +150,9 @@ protected:
                 sf = m.getComptonNormalizedScatterFactor(qApp);
                 qApp += sf < T { 0.5 } ? T { 0.5 } : T { 0.1 };
             }
-            m_comptonScatterFactor.emplace_back(T { 0 }, qApp, [&](const T q) -> T { return m.getComptonNormalizedScatterFactor(q); });
+            m_comptonScatterFactor.push_back(cachedTable<CubicSplineInterpolator<T, SPLINE_NODES>>(1, m, qApp, [&] {
+                return CubicSplineInterpolator<T, SPLINE_NODES>(T { 0 }, qApp, [&](const T q) -> T { return m.getComptonNormalizedScatterFactor(q); });
+            }));
         }
     }
 

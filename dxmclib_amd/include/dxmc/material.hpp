@@ -65,6 +65,9 @@ public:
 
     bool isValid(void) const { return m_valid && m_hasDensity; }
     const std::string& name(void) const { return m_name; }
+    // the composition as text (atomic numbers and the bits of the number
+    // fractions): equal keys mean equal physics data
+    std::string compositionKey(void) const;
     const std::string& prettyName(void) const;
 
     bool hasStandardDensity(void) const { return m_hasDensity; }

@@ -25,6 +25,8 @@
 #include <array>
 #include <cctype>
 #include <cmath>
+#include <cstdio>
+#include <cstring>
 #include <map>
 #include <mutex>
 #include <numeric>
@@ -544,6 +546,19 @@ Material::Material(const std::vector<int>& elements, const std::vector<double>& 
 }
 
 const std::string& Material::prettyName(void) const { return m_prettyName.empty() ? m_name : m_prettyName; }
+
+std::string Material::compositionKey(void) const
+{
+    std::string k;
+    char buf[48];
+    for (std::size_t i = 0; i < m_elements.size(); ++i) {
+        std::uint64_t bits;
+        std::memcpy(&bits, &m_elementNumberFraction[i], sizeof(bits));
+        std::snprintf(buf, sizeof(buf), "%d:%016llx;", m_elements[i], static_cast<unsigned long long>(bits));
+        k += buf;
+    }
+    return k;
+}
 
 void Material::setStandardDensity(double density)
 {
