@@ -213,6 +213,12 @@ class Device:
         self._check(self.lib.dxmc_hip_world_bytes(h, C.byref(b)))
         return b.value
 
+    def world_max_density(self, h, n_materials):
+        """Per-material maximum density, from the device scan at upload."""
+        out = np.zeros(n_materials, dtype=np.float64)
+        self._check(self.lib.dxmc_hip_world_max_density(h, _ptr(out, C.c_double), n_materials))
+        return out
+
     def destroy_world(self, h):
         self._check(self.lib.dxmc_hip_world_destroy(self.ctx, h))
 

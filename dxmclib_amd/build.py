@@ -38,6 +38,7 @@ HIP_SOURCES = {
     "tables.hip": ["-ffp-contract=off"],
     "transport_fp32.hip": ["-fgpu-approx-transcendentals", "-fno-hip-fp32-correctly-rounded-divide-sqrt"],
     "autotune.hip": [],
+    "world.hip": [],
 }
 HOST_SOURCES = ["src/material.cpp", "src/host_capi.cpp", "src/server.cpp"]
 HOST_FLAGS = ["-O2", "-std=c++20", "-fPIC", "-ffp-contract=off",

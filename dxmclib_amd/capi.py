@@ -254,6 +254,8 @@ class RequestInfo(C.Structure):
         ("device_bytes", C.c_uint64),
         ("status", C.c_int32),
         ("slot", C.c_int32),
+        ("uploaded", C.c_double),
+        ("transported", C.c_double),
     ]
 
 
@@ -274,6 +276,7 @@ HIP_SYMBOLS = {
     "dxmc_hip_world_upload": (C.c_int, [C.c_void_p, C.POINTER(WorldDesc), C.POINTER(C.c_void_p)]),
     "dxmc_hip_world_destroy": (C.c_int, [C.c_void_p, C.c_void_p]),
     "dxmc_hip_world_bytes": (C.c_int, [C.c_void_p, C.POINTER(C.c_uint64)]),
+    "dxmc_hip_world_max_density": (C.c_int, [C.c_void_p, C.POINTER(C.c_double), C.c_uint32]),
     "dxmc_hip_tables_upload": (C.c_int, [C.c_void_p, C.POINTER(TablesDesc), C.POINTER(C.c_void_p)]),
     "dxmc_hip_tables_destroy": (C.c_int, [C.c_void_p, C.c_void_p]),
     "dxmc_hip_tally_create": (C.c_int, [C.c_void_p, C.c_uint64, C.POINTER(C.c_void_p)]),

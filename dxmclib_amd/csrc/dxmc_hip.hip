@@ -172,57 +172,6 @@ __global__ void fillZero(double* a, uint32_t* b, double* c, uint64_t n)
     }
 }
 
-__global__ void packVoxels(const uint8_t* mat, const double* dens, const uint8_t* meas, uint2* vox, uint64_t n)
-{
-    const uint64_t i = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (i < n) {
-        uint2 v;
-        v.x = __float_as_uint(static_cast<float>(dens[i]));
-        v.y = static_cast<uint32_t>(mat[i]) | (meas ? static_cast<uint32_t>(meas[i]) << 8 : 0u);
-        vox[i] = v;
-    }
-}
-
-// Distinct FP32 voxel records (density bits, material | measurement << 8)
-// in order of first appearance, and the palette index of every voxel.
-// Returns false (palette unusable) beyond 65536 distinct records.
-bool buildPalette(const uint8_t* mat, const double* dens, const uint8_t* meas, uint64_t n, std::vector<uint2>& palette,
-    std::vector<uint16_t>& index)
-{
-    constexpr uint32_t limit = 65536;
-    constexpr uint32_t capacity = 1u << 18; // open addressing, load <= 1/4
-    std::vector<uint64_t> keys(capacity, ~0ULL);
-    std::vector<uint32_t> slots(capacity, 0);
-    palette.clear();
-    index.resize(n);
-    uint64_t lastKey = ~0ULL;
-    uint16_t lastIndex = 0;
-    for (uint64_t i = 0; i < n; ++i) {
-        const uint32_t bits = std::bit_cast<uint32_t>(static_cast<float>(dens[i]));
-        const uint32_t tag = static_cast<uint32_t>(mat[i]) | (meas ? static_cast<uint32_t>(meas[i]) << 8 : 0u);
-        const uint64_t key = static_cast<uint64_t>(tag) << 32 | bits;
-        if (key != lastKey) {
-            uint32_t h = static_cast<uint32_t>((key * 0x9E3779B97F4A7C15ULL) >> 46);
-            while (keys[h] != key && keys[h] != ~0ULL)
-                h = (h + 1) & (capacity - 1);
-            if (keys[h] == ~0ULL) {
-                if (palette.size() == limit) {
-                    palette.clear();
-                    index.clear();
-                    return false;
-                }
-                keys[h] = key;
-                slots[h] = static_cast<uint32_t>(palette.size());
-                palette.push_back(make_uint2(bits, tag));
-            }
-            lastKey = key;
-            lastIndex = static_cast<uint16_t>(slots[h]);
-        }
-        index[i] = lastIndex;
-    }
-    return true;
-}
-
 uint32_t gridFor(uint64_t n, uint32_t block) { return static_cast<uint32_t>((n + block - 1) / block); }
 
 template <typename T>
@@ -365,119 +314,7 @@ int dxmc_hip_synchronize(dxmc_ctx* c)
     return DXMC_OK;
 }
 
-// ------------------------------------------------------------------ world
-
-int dxmc_hip_world_upload(dxmc_ctx* c, const dxmc_world_desc* d, dxmc_world** out)
-{
-    if (!c || !d || !out)
-        return DXMC_ERR_ARGUMENT;
-    *out = nullptr;
-    const uint64_t n = d->dims[0] * d->dims[1] * d->dims[2];
-    if (n == 0 || !d->material || (!d->density_f32 == !d->density_f64) || d->n_materials == 0 || d->n_materials > 256)
-        return c->fail(DXMC_ERR_ARGUMENT, "world_upload: need dims, material, exactly one density array and 1..256 materials");
-    if (d->dims[0] > 0xffffffffULL || d->dims[1] > 0xffffffffULL || d->dims[2] > 0xffffffffULL)
-        return c->fail(DXMC_ERR_ARGUMENT, "world_upload: dimension exceeds 2^32-1");
-    for (int i = 0; i < 3; ++i)
-        if (!(d->spacing[i] > 0) || !(d->extent_safe[2 * i] < d->extent_safe[2 * i + 1]))
-            return c->fail(DXMC_ERR_ARGUMENT, "world_upload: invalid spacing or extent");
-    // material indices must address the material table (reference World::validate)
-    uint8_t maxMat = 0;
-    for (uint64_t i = 0; i < n; ++i)
-        maxMat = std::max(maxMat, d->material[i]);
-    if (maxMat >= d->n_materials)
-        return c->fail(DXMC_ERR_ARGUMENT, "world_upload: material index outside the material map");
-    DXMC_CHECK(c, hipSetDevice(c->device));
-    auto* w = new dxmc_world();
-    w->ctx = c;
-    w->n = n;
-    for (int i = 0; i < 3; ++i) {
-        w->dims[i] = static_cast<uint32_t>(d->dims[i]);
-        w->spacing[i] = d->spacing[i];
-    }
-    for (int i = 0; i < 6; ++i)
-        w->extent[i] = d->extent_safe[i];
-    w->nMaterials = d->n_materials;
-    w->hasMeasurement = false;
-    if (d->measurement)
-        for (uint64_t i = 0; i < n && !w->hasMeasurement; ++i)
-            w->hasMeasurement = d->measurement[i] != 0;
-    std::vector<double> dens64;
-    const double* densSrc = d->density_f64;
-    if (!densSrc) {
-        dens64.resize(n);
-        for (uint64_t i = 0; i < n; ++i)
-            dens64[i] = static_cast<double>(d->density_f32[i]);
-        densSrc = dens64.data();
-    }
-    w->maxDensity.assign(d->n_materials, 0.0);
-    for (uint64_t i = 0; i < n; ++i)
-        w->maxDensity[d->material[i]] = std::max(w->maxDensity[d->material[i]], densSrc[i]);
-    // FP32 voxel records: palette of the distinct (density, material,
-    // measurement) records and a 1- or 2-byte index per voxel when there
-    // are at most 65536 of them (exact: the palette holds the same FP32
-    // record the voxel would)
-    std::vector<uint2> palette;
-    std::vector<uint16_t> index;
-    const bool indexed = buildPalette(d->material, densSrc, w->hasMeasurement ? d->measurement : nullptr, n, palette, index);
-    w->voxelBytes = !indexed ? 8u : palette.size() <= 256 ? 1u : 2u;
-    w->nPalette = indexed ? static_cast<uint32_t>(palette.size()) : 0u;
-    int st = DXMC_OK;
-    if ((st = c->alloc(&w->material, n)) || (st = c->alloc(&w->density, n * sizeof(double))) || (st = c->alloc(&w->voxels, n * w->voxelBytes))
-        || (indexed && (st = c->alloc(&w->palette, palette.size() * sizeof(uint2)))) || (w->hasMeasurement && (st = c->alloc(&w->measurement, n)))) {
-        dxmc_hip_world_destroy(c, w);
-        return st;
-    }
-    if ((st = c->copy(w->material, d->material, n)) || (st = c->copy(w->density, densSrc, n * sizeof(double)))
-        || (w->hasMeasurement && (st = c->copy(w->measurement, d->measurement, n)))
-        || (indexed && (st = c->copy(w->palette, palette.data(), palette.size() * sizeof(uint2))))) {
-        dxmc_hip_world_destroy(c, w);
-        return st;
-    }
-    if (w->voxelBytes == 2) {
-        st = c->copy(w->voxels, index.data(), n * sizeof(uint16_t));
-    } else if (w->voxelBytes == 1) {
-        std::vector<uint8_t> narrow(index.begin(), index.end());
-        st = c->copy(w->voxels, narrow.data(), n);
-    } else {
-        hipLaunchKernelGGL(packVoxels, dim3(gridFor(n, 256)), dim3(256), 0, c->stream, static_cast<const uint8_t*>(w->material),
-            static_cast<const double*>(w->density), static_cast<const uint8_t*>(w->measurement), static_cast<uint2*>(w->voxels), n);
-        st = c->check(hipGetLastError());
-    }
-    if (st || c->check(hipStreamSynchronize(c->stream))) {
-        dxmc_hip_world_destroy(c, w);
-        return st ? st : DXMC_ERR_DEVICE;
-    }
-    w->bytes = n * (1 + sizeof(double) + w->voxelBytes + (w->hasMeasurement ? 1 : 0)) + w->nPalette * sizeof(uint2);
-    *out = w;
-    return DXMC_OK;
-}
-
-int dxmc_hip_world_destroy(dxmc_ctx* c, dxmc_world* w)
-{
-    if (!c || !w)
-        return DXMC_ERR_ARGUMENT;
-    if (w->ctx != c)
-        return c->fail(DXMC_ERR_STATE, "world_destroy: world belongs to another context");
-    std::erase_if(c->tuned, [w](const auto& kv) { return std::get<0>(kv.first) == w; });
-    hipSetDevice(c->device);
-    hipStreamSynchronize(c->stream);
-    c->release(w->material);
-    c->release(w->density);
-    c->release(w->measurement);
-    c->release(w->voxels);
-    c->release(w->palette);
-    hipStreamSynchronize(c->stream);
-    delete w;
-    return DXMC_OK;
-}
-
-int dxmc_hip_world_bytes(const dxmc_world* w, uint64_t* bytes)
-{
-    if (!w || !bytes)
-        return DXMC_ERR_ARGUMENT;
-    *bytes = w->bytes;
-    return DXMC_OK;
-}
+// World upload, destroy and queries: world.hip
 
 // ----------------------------------------------------------------- tables
 

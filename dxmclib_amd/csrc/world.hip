@@ -1,0 +1,382 @@
+// World upload with its voxel passes on the device.
+//
+// A world arrives as plain host arrays (material index, density, optional
+// measurement flags). After one host->device copy of each array, kernels do
+// every pass over the voxels:
+//   scanWorld<D>    largest material index (World::validate, reference
+//                   world.hpp:182-227), per-material maximum density (the
+//                   majorant's input, reference attenuationinterpolator.hpp
+//                   :52-58), whether any measurement flag is set, and the
+//                   FP64 density copy the normalisation reads
+//   paletteInsert   distinct FP32 voxel records (density bits, material |
+//                   measurement << 8) into an open-addressing hash table
+//   paletteCompact  palette index of each distinct record
+//   paletteIndex<I> 1- or 2-byte palette index of every voxel
+//   packVoxels      8-byte records when there are more than 65536
+//                   distinct records
+// The host reads back a few hundred bytes (maxima, flags, distinct count),
+// so the cost of an upload is the copy and a few HBM passes.
+#include "dxmc_hip.h"
+#include "internal.hpp"
+
+#include <algorithm>
+#include <string>
+#include <type_traits>
+#include <vector>
+
+using namespace dxmc_backend;
+
+namespace {
+
+constexpr uint32_t SCAN_BLOCK = 256; // one thread per material slot in scanWorld
+constexpr uint32_t SCAN_GRID = 4096; // grid-stride blocks of the voxel passes
+constexpr uint32_t INSERT_GRID = 1024; // fewer in flight: bounds the table's fill
+constexpr uint32_t PALETTE_LIMIT = 65536;
+constexpr uint32_t HASH_BITS = 20;
+constexpr uint32_t HASH_SLOTS = 1u << HASH_BITS;
+constexpr uint32_t HASH_PROBES = 256; // longer probe sequences fall back to 8-byte records
+constexpr uint32_t COMPACT_BLOCK = 1024;
+constexpr unsigned long long EMPTY = ~0ull; // not a record key: the tag is < 2^16
+
+uint32_t gridFor(uint64_t n, uint32_t block, uint32_t cap) { return static_cast<uint32_t>(std::min<uint64_t>((n + block - 1) / block, cap)); }
+
+// Order-preserving image of a double: unsigned comparison of images is the
+// numeric order. 0 is below the image of every number (only a negative NaN
+// maps to 0, and NaNs are skipped).
+__host__ __device__ inline unsigned long long orderKey(double v)
+{
+    const unsigned long long b = __builtin_bit_cast(unsigned long long, v);
+    return (b >> 63) ? ~b : (b | (1ull << 63));
+}
+
+double fromOrderKey(unsigned long long k)
+{
+    if (k == 0)
+        return 0.0;
+    return __builtin_bit_cast(double, (k >> 63) ? (k & ~(1ull << 63)) : ~k);
+}
+
+__device__ inline unsigned long long recordKey(const uint8_t* mat, const double* dens, const uint8_t* meas, uint64_t i)
+{
+    const uint32_t tag = static_cast<uint32_t>(mat[i]) | (meas ? static_cast<uint32_t>(meas[i]) << 8 : 0u);
+    return static_cast<unsigned long long>(tag) << 32 | __float_as_uint(static_cast<float>(dens[i]));
+}
+
+__device__ inline uint32_t hashSlot(unsigned long long k) { return static_cast<uint32_t>((k * 0x9E3779B97F4A7C15ull) >> (64 - HASH_BITS)); }
+
+// flags[0]: largest material index; flags[1]: any measurement flag set.
+// maxKey[m]: order key of material m's largest density (0: no voxel).
+template <typename D>
+__global__ __launch_bounds__(SCAN_BLOCK) void scanWorld(const uint8_t* __restrict__ mat, const D* __restrict__ dens, double* __restrict__ dens64,
+    const uint8_t* __restrict__ meas, uint64_t n, unsigned long long* __restrict__ maxKey, unsigned int* __restrict__ flags)
+{
+    __shared__ unsigned long long key[SCAN_BLOCK];
+    __shared__ unsigned int blockMat, blockMeas;
+    key[threadIdx.x] = 0;
+    if (threadIdx.x == 0) {
+        blockMat = 0;
+        blockMeas = 0;
+    }
+    __syncthreads();
+    uint32_t lastMat = SCAN_BLOCK, largestMat = 0, anyMeas = 0;
+    unsigned long long lastKey = 0;
+    const uint64_t stride = static_cast<uint64_t>(gridDim.x) * SCAN_BLOCK;
+    for (uint64_t i = static_cast<uint64_t>(blockIdx.x) * SCAN_BLOCK + threadIdx.x; i < n; i += stride) {
+        const double v = static_cast<double>(dens[i]);
+        if constexpr (!std::is_same_v<D, double>)
+            dens64[i] = v;
+        const uint32_t m = mat[i];
+        largestMat = max(largestMat, m);
+        if (meas)
+            anyMeas |= meas[i];
+        if (v == v) {
+            const unsigned long long k = orderKey(v);
+            // skip the LDS atomic when this thread already posted a larger
+            // value for the same material
+            if (m != lastMat || k > lastKey) {
+                atomicMax(&key[m], k);
+                lastMat = m;
+                lastKey = k;
+            }
+        }
+    }
+    atomicMax(&blockMat, largestMat);
+    if (anyMeas)
+        atomicOr(&blockMeas, 1u);
+    __syncthreads();
+    if (key[threadIdx.x])
+        atomicMax(&maxKey[threadIdx.x], key[threadIdx.x]);
+    if (threadIdx.x == 0) {
+        atomicMax(&flags[0], blockMat);
+        if (blockMeas)
+            atomicOr(&flags[1], 1u);
+    }
+}
+
+// state[0]: distinct records inserted; state[1]: overflow (too many
+// records, or a probe sequence longer than HASH_PROBES). Every thread stops
+// at its next voxel once state[1] is set.
+__global__ __launch_bounds__(SCAN_BLOCK) void paletteInsert(const uint8_t* __restrict__ mat, const double* __restrict__ dens,
+    const uint8_t* __restrict__ meas, uint64_t n, unsigned long long* keys, unsigned int* state)
+{
+    unsigned long long last = EMPTY;
+    const uint64_t stride = static_cast<uint64_t>(gridDim.x) * SCAN_BLOCK;
+    for (uint64_t i = static_cast<uint64_t>(blockIdx.x) * SCAN_BLOCK + threadIdx.x; i < n; i += stride) {
+        if (__hip_atomic_load(&state[1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))
+            return;
+        const unsigned long long k = recordKey(mat, dens, meas, i);
+        if (k == last)
+            continue;
+        last = k;
+        uint32_t h = hashSlot(k);
+        bool placed = false;
+        for (uint32_t p = 0; p < HASH_PROBES; ++p, h = (h + 1) & (HASH_SLOTS - 1)) {
+            const unsigned long long cur = __hip_atomic_load(&keys[h], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            if (cur == k) {
+                placed = true;
+                break;
+            }
+            if (cur != EMPTY)
+                continue;
+            const unsigned long long prev = atomicCAS(&keys[h], EMPTY, k);
+            if (prev == EMPTY) {
+                if (atomicAdd(&state[0], 1u) >= PALETTE_LIMIT)
+                    atomicOr(&state[1], 1u);
+                placed = true;
+                break;
+            }
+            if (prev == k) {
+                placed = true;
+                break;
+            }
+        }
+        if (!placed) {
+            atomicOr(&state[1], 1u);
+            return;
+        }
+    }
+}
+
+// One block: thread t owns slots t, t + 1024, ...; ranks follow (t, slot).
+__global__ __launch_bounds__(COMPACT_BLOCK) void paletteCompact(const unsigned long long* __restrict__ keys, uint32_t* __restrict__ rank,
+    uint2* __restrict__ palette)
+{
+    __shared__ uint32_t count[COMPACT_BLOCK];
+    const uint32_t t = threadIdx.x;
+    uint32_t c = 0;
+    for (uint32_t j = t; j < HASH_SLOTS; j += COMPACT_BLOCK)
+        c += keys[j] != EMPTY;
+    count[t] = c;
+    __syncthreads();
+    for (uint32_t off = 1; off < COMPACT_BLOCK; off <<= 1) {
+        const uint32_t v = t >= off ? count[t - off] : 0u;
+        __syncthreads();
+        count[t] += v;
+        __syncthreads();
+    }
+    uint32_t r = count[t] - c;
+    for (uint32_t j = t; j < HASH_SLOTS; j += COMPACT_BLOCK) {
+        const unsigned long long k = keys[j];
+        if (k != EMPTY) {
+            rank[j] = r;
+            palette[r] = make_uint2(static_cast<uint32_t>(k), static_cast<uint32_t>(k >> 32));
+            ++r;
+        }
+    }
+}
+
+// Every key is in the table (no overflow), within HASH_PROBES of its slot.
+// rank[] is zero-filled first, so even a miss would yield index 0.
+template <typename I>
+__global__ __launch_bounds__(SCAN_BLOCK) void paletteIndex(const uint8_t* __restrict__ mat, const double* __restrict__ dens,
+    const uint8_t* __restrict__ meas, uint64_t n, const unsigned long long* __restrict__ keys, const uint32_t* __restrict__ rank, I* __restrict__ out)
+{
+    unsigned long long last = EMPTY;
+    I index = 0;
+    const uint64_t stride = static_cast<uint64_t>(gridDim.x) * SCAN_BLOCK;
+    for (uint64_t i = static_cast<uint64_t>(blockIdx.x) * SCAN_BLOCK + threadIdx.x; i < n; i += stride) {
+        const unsigned long long k = recordKey(mat, dens, meas, i);
+        if (k != last) {
+            uint32_t h = hashSlot(k);
+            for (uint32_t p = 1; p < HASH_PROBES && keys[h] != k; ++p)
+                h = (h + 1) & (HASH_SLOTS - 1);
+            index = static_cast<I>(rank[h]);
+            last = k;
+        }
+        out[i] = index;
+    }
+}
+
+__global__ __launch_bounds__(SCAN_BLOCK) void packVoxels(const uint8_t* __restrict__ mat, const double* __restrict__ dens,
+    const uint8_t* __restrict__ meas, uint2* __restrict__ vox, uint64_t n)
+{
+    const uint64_t stride = static_cast<uint64_t>(gridDim.x) * SCAN_BLOCK;
+    for (uint64_t i = static_cast<uint64_t>(blockIdx.x) * SCAN_BLOCK + threadIdx.x; i < n; i += stride) {
+        uint2 v;
+        v.x = __float_as_uint(static_cast<float>(dens[i]));
+        v.y = static_cast<uint32_t>(mat[i]) | (meas ? static_cast<uint32_t>(meas[i]) << 8 : 0u);
+        vox[i] = v;
+    }
+}
+
+// Scan results read back by the host.
+struct ScanResult {
+    unsigned long long maxKey[SCAN_BLOCK];
+    unsigned int flags[2];
+    unsigned int palette[2]; // distinct records, overflow
+};
+
+int fillVoxelRecords(dxmc_ctx* c, dxmc_world* w, ScanResult* dev, ScanResult& host)
+{
+    const uint64_t n = w->n;
+    const auto* mat = static_cast<const uint8_t*>(w->material);
+    const auto* dens = static_cast<const double*>(w->density);
+    const auto* meas = static_cast<const uint8_t*>(w->measurement);
+    DeviceBuffer keys(c), rank(c);
+    int st = DXMC_OK;
+    if ((st = keys.alloc(HASH_SLOTS * sizeof(unsigned long long))) || (st = rank.zeros(HASH_SLOTS * sizeof(uint32_t)))
+        || (st = c->check(hipMemsetAsync(keys.p, 0xff, HASH_SLOTS * sizeof(unsigned long long), c->stream))))
+        return st;
+    auto* k = static_cast<unsigned long long*>(keys.p);
+    hipLaunchKernelGGL(paletteInsert, dim3(gridFor(n, SCAN_BLOCK, INSERT_GRID)), dim3(SCAN_BLOCK), 0, c->stream, mat, dens, meas, n, k, dev->palette);
+    if ((st = c->check(hipGetLastError())) || (st = c->download(host.palette, dev->palette, sizeof(host.palette))))
+        return st;
+    const uint32_t distinct = host.palette[0];
+    if (host.palette[1] == 0 && distinct <= PALETTE_LIMIT) {
+        w->voxelBytes = distinct <= 256 ? 1u : 2u;
+        w->nPalette = distinct;
+        if ((st = c->alloc(&w->palette, distinct * sizeof(uint2))) || (st = c->alloc(&w->voxels, n * w->voxelBytes)))
+            return st;
+        auto* r = static_cast<uint32_t*>(rank.p);
+        hipLaunchKernelGGL(paletteCompact, dim3(1), dim3(COMPACT_BLOCK), 0, c->stream, k, r, static_cast<uint2*>(w->palette));
+        if (w->voxelBytes == 1)
+            hipLaunchKernelGGL(paletteIndex<uint8_t>, dim3(gridFor(n, SCAN_BLOCK, SCAN_GRID)), dim3(SCAN_BLOCK), 0, c->stream, mat, dens, meas, n, k, r,
+                static_cast<uint8_t*>(w->voxels));
+        else
+            hipLaunchKernelGGL(paletteIndex<uint16_t>, dim3(gridFor(n, SCAN_BLOCK, SCAN_GRID)), dim3(SCAN_BLOCK), 0, c->stream, mat, dens, meas, n, k, r,
+                static_cast<uint16_t*>(w->voxels));
+    } else {
+        w->voxelBytes = 8;
+        w->nPalette = 0;
+        if ((st = c->alloc(&w->voxels, n * sizeof(uint2))))
+            return st;
+        hipLaunchKernelGGL(packVoxels, dim3(gridFor(n, SCAN_BLOCK, SCAN_GRID)), dim3(SCAN_BLOCK), 0, c->stream, mat, dens, meas,
+            static_cast<uint2*>(w->voxels), n);
+    }
+    if ((st = c->check(hipGetLastError())))
+        return st;
+    // the temporaries are released on the stream after these kernels
+    return c->check(hipStreamSynchronize(c->stream));
+}
+
+} // namespace
+
+extern "C" {
+
+int dxmc_hip_world_upload(dxmc_ctx* c, const dxmc_world_desc* d, dxmc_world** out)
+{
+    if (!c || !d || !out)
+        return DXMC_ERR_ARGUMENT;
+    *out = nullptr;
+    const uint64_t n = d->dims[0] * d->dims[1] * d->dims[2];
+    if (n == 0 || !d->material || (!d->density_f32 == !d->density_f64) || d->n_materials == 0 || d->n_materials > 256)
+        return c->fail(DXMC_ERR_ARGUMENT, "world_upload: need dims, material, exactly one density array and 1..256 materials");
+    if (d->dims[0] > 0xffffffffULL || d->dims[1] > 0xffffffffULL || d->dims[2] > 0xffffffffULL)
+        return c->fail(DXMC_ERR_ARGUMENT, "world_upload: dimension exceeds 2^32-1");
+    for (int i = 0; i < 3; ++i)
+        if (!(d->spacing[i] > 0) || !(d->extent_safe[2 * i] < d->extent_safe[2 * i + 1]))
+            return c->fail(DXMC_ERR_ARGUMENT, "world_upload: invalid spacing or extent");
+    DXMC_CHECK(c, hipSetDevice(c->device));
+    auto* w = new dxmc_world();
+    w->ctx = c;
+    w->n = n;
+    for (int i = 0; i < 3; ++i) {
+        w->dims[i] = static_cast<uint32_t>(d->dims[i]);
+        w->spacing[i] = d->spacing[i];
+    }
+    for (int i = 0; i < 6; ++i)
+        w->extent[i] = d->extent_safe[i];
+    w->nMaterials = d->n_materials;
+    auto fail = [&](int st) {
+        dxmc_hip_world_destroy(c, w);
+        return st;
+    };
+    DeviceBuffer dens32(c), scan(c);
+    int st = DXMC_OK;
+    if ((st = c->alloc(&w->material, n)) || (st = c->alloc(&w->density, n * sizeof(double))) || (d->measurement && (st = c->alloc(&w->measurement, n)))
+        || (st = scan.zeros(sizeof(ScanResult))))
+        return fail(st);
+    if ((st = c->copy(w->material, d->material, n)) || (d->measurement && (st = c->copy(w->measurement, d->measurement, n))))
+        return fail(st);
+    auto* res = static_cast<ScanResult*>(scan.p);
+    const auto* mat = static_cast<const uint8_t*>(w->material);
+    const auto* meas = static_cast<const uint8_t*>(w->measurement);
+    auto* dens64 = static_cast<double*>(w->density);
+    const dim3 grid(gridFor(n, SCAN_BLOCK, SCAN_GRID)), block(SCAN_BLOCK);
+    if (d->density_f64) {
+        if ((st = c->copy(w->density, d->density_f64, n * sizeof(double))))
+            return fail(st);
+        hipLaunchKernelGGL(scanWorld<double>, grid, block, 0, c->stream, mat, dens64, dens64, meas, n, res->maxKey, res->flags);
+    } else {
+        if ((st = dens32.upload(d->density_f32, n * sizeof(float))))
+            return fail(st);
+        hipLaunchKernelGGL(scanWorld<float>, grid, block, 0, c->stream, mat, static_cast<const float*>(dens32.p), dens64, meas, n, res->maxKey,
+            res->flags);
+    }
+    ScanResult host {};
+    if ((st = c->check(hipGetLastError())) || (st = c->download(&host, res, sizeof(ScanResult))))
+        return fail(st);
+    if (host.flags[0] >= d->n_materials)
+        return fail(c->fail(DXMC_ERR_ARGUMENT, "world_upload: material index outside the material map"));
+    w->hasMeasurement = host.flags[1] != 0;
+    if (!w->hasMeasurement && w->measurement) {
+        c->release(w->measurement);
+        w->measurement = nullptr;
+    }
+    // the reference starts each maximum at 0 (attenuationinterpolator.hpp:52-58)
+    w->maxDensity.resize(d->n_materials);
+    for (uint32_t m = 0; m < d->n_materials; ++m)
+        w->maxDensity[m] = std::max(0.0, fromOrderKey(host.maxKey[m]));
+    if ((st = fillVoxelRecords(c, w, res, host)))
+        return fail(st);
+    w->bytes = n * (1 + sizeof(double) + w->voxelBytes + (w->hasMeasurement ? 1 : 0)) + w->nPalette * sizeof(uint2);
+    *out = w;
+    return DXMC_OK;
+}
+
+int dxmc_hip_world_destroy(dxmc_ctx* c, dxmc_world* w)
+{
+    if (!c || !w)
+        return DXMC_ERR_ARGUMENT;
+    if (w->ctx != c)
+        return c->fail(DXMC_ERR_STATE, "world_destroy: world belongs to another context");
+    std::erase_if(c->tuned, [w](const auto& kv) { return std::get<0>(kv.first) == w; });
+    hipSetDevice(c->device);
+    hipStreamSynchronize(c->stream);
+    c->release(w->material);
+    c->release(w->density);
+    c->release(w->measurement);
+    c->release(w->voxels);
+    c->release(w->palette);
+    hipStreamSynchronize(c->stream);
+    delete w;
+    return DXMC_OK;
+}
+
+int dxmc_hip_world_bytes(const dxmc_world* w, uint64_t* bytes)
+{
+    if (!w || !bytes)
+        return DXMC_ERR_ARGUMENT;
+    *bytes = w->bytes;
+    return DXMC_OK;
+}
+
+int dxmc_hip_world_max_density(const dxmc_world* w, double* out, uint32_t n)
+{
+    if (!w || !out || n != w->nMaterials)
+        return DXMC_ERR_ARGUMENT;
+    std::copy(w->maxDensity.begin(), w->maxDensity.end(), out);
+    return DXMC_OK;
+}
+
+} // extern "C"

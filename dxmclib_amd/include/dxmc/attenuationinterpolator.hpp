@@ -75,6 +75,14 @@ public:
 
     void generateFromMaxDensity(const std::vector<Material>& materials, const std::vector<T>& maxDens, const T maxEnergy, const T minEnergy)
     {
+        generateMaterials(materials, maxEnergy, minEnergy);
+        generateMajorant(maxDens);
+    }
+
+    // The grid and the per-material coefficients: everything but the
+    // majorant, which needs each material's maximum voxel density.
+    void generateMaterials(const std::vector<Material>& materials, const T maxEnergy, const T minEnergy)
+    {
         m_resolution = std::max(static_cast<std::size_t>(maxEnergy - minEnergy), std::size_t { 10 });
         const T logMin = std::log10(minEnergy);
         const T logMax = std::log10(maxEnergy);
@@ -119,7 +127,7 @@ public:
             }
         }
 
-        const T firstNode = x.front();
+        m_firstNode = x.front();
         m_x.assign(x.begin() + 1, x.end());
         if (!edges.empty()) {
             const auto up = std::upper_bound(m_x.cbegin(), m_x.cend(), edges.back());
@@ -138,11 +146,15 @@ public:
             m_linearEnergy = m_x.front();
         }
         m_resolution = m_x.size();
+        m_maxCoefficients.clear();
+    }
 
-        // majorant: 1 / max_m(maxDens_m * sum of the three coefficients) at
-        // every node (including the dropped first one), fitted log-log again
+    // majorant: 1 / max_m(maxDens_m * sum of the three coefficients) at
+    // every node (including the dropped first one), fitted log-log again
+    void generateMajorant(const std::vector<T>& maxDens)
+    {
         std::vector<T> xm = m_x;
-        xm.insert(xm.begin(), firstNode);
+        xm.insert(xm.begin(), m_firstNode);
         std::vector<T> ym(xm.size());
         for (std::size_t i = 0; i < xm.size(); ++i) {
             T worst = 0;
@@ -212,5 +224,6 @@ private:
     std::size_t m_linearIndex = 0;
     T m_linearStep = 0;
     T m_linearEnergy = 0;
+    T m_firstNode = 0;
 };
 }

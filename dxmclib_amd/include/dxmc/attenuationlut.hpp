@@ -47,11 +47,20 @@ public:
     // same, from precomputed per-material maximum densities (resident worlds)
     void generate(const std::vector<Material>& materials, const std::vector<T>& maxDensity, T maxEnergy, T minEnergy)
     {
-        generate(materials, maxEnergy, minEnergy, false);
-        AttenuationLutInterpolator<T> lut;
-        lut.generateFromMaxDensity(materials, maxDensity, std::max(m_maxEnergy, T { 50 }), std::max(T { 0.1 }, m_minEnergy));
-        m_attenuationData = lut;
+        generateWorldTables(materials, maxEnergy, minEnergy);
+        setMaxDensity(maxDensity);
     }
+
+    // A world's tables without the majorant; setMaxDensity() adds it once
+    // the per-material maximum densities are known (the device computes
+    // them when it uploads the world).
+    void generateWorldTables(const std::vector<Material>& materials, T maxEnergy, T minEnergy)
+    {
+        generate(materials, maxEnergy, minEnergy, false);
+        m_attenuationData = AttenuationLutInterpolator<T>();
+        m_attenuationData.generateMaterials(materials, std::max(m_maxEnergy, T { 50 }), std::max(T { 0.1 }, m_minEnergy));
+    }
+    void setMaxDensity(const std::vector<T>& maxDensity) { m_attenuationData.generateMajorant(maxDensity); }
 
     void generate(const std::vector<Material>& materials, T maxEnergy = 150, T minEnergy = 1, bool generatePhotonData = true)
     {

@@ -116,14 +116,22 @@ public:
     int devicePrecision() const { return m_precision; }
     void setRunConfiguration(const dxmc_run_cfg& cfg) { m_tuning = cfg; }
     const dxmc_run_stats& lastRunStats() const { return m_stats; }
+    // wall time of the device phases of the last run [s]: world, tables and
+    // tally to the device; transport and normalisation; tally to the host
+    struct DeviceTiming {
+        double upload = 0, transport = 0, download = 0;
+    };
+    const DeviceTiming& lastDeviceTiming() const { return m_timing; }
     const std::string& lastError() const { return m_error; }
 
     // The host part of operator(): attenuation tables for the world's
-    // materials, the source's exposure table aligned to the world basis and
-    // the FP64 density copy handed to the device.
+    // materials and the source's exposure table aligned to the world basis.
+    // No pass over the voxels: the Woodcock majorant needs each material's
+    // maximum density, which the device computes when execute() uploads the
+    // world; the majorant is then fitted on the host from those values.
     struct Prepared {
         bool valid = false;
-        std::vector<double> density;
+        AttenuationLut<double> lut; // all tables but the majorant
         TableSet tables;
         std::vector<dxmc_exposure> exposures;
         std::uint64_t histories = 0;
@@ -142,11 +150,8 @@ public:
             return p;
         p.histories = source->historiesPerExposure() * source->totalExposures();
         // host tables (always FP64; the device derives its FP32 copy)
-        p.density.assign(world.densityArray()->cbegin(), world.densityArray()->cend());
-        const auto maxDens = AttenuationLutInterpolator<double>::maxDensities(world.materialMap().size(), p.density.cbegin(), p.density.cend(),
-            world.materialIndexArray()->cbegin());
-        m_attenuationLut.generate(world.materialMap(), maxDens, static_cast<double>(source->maxPhotonEnergyProduced()), 1.0);
-        p.tables.setMaterialTables(m_attenuationLut);
+        p.lut.generateWorldTables(world.materialMap(), static_cast<double>(source->maxPhotonEnergyProduced()), 1.0);
+        p.tables.setMaterialTables(p.lut);
         ExposureTable<T> exposures(p.tables);
         const auto& basis = world.directionCosines();
         const std::uint64_t nExp = source->totalExposures();
@@ -178,7 +183,24 @@ public:
             progressbar->setDoseData(result.dose.data(), world.dimensions(), world.spacing());
         }
         const auto start = std::chrono::steady_clock::now();
-        std::vector<double> dose(world.size()), variance(world.size());
+        // FP64 tallies go straight into the result when T is double
+        std::vector<double> dose, variance;
+        if constexpr (!std::is_same_v<T, double>) {
+            dose.resize(world.size());
+            variance.resize(world.size());
+        }
+        std::vector<double>& doseOut = [&]() -> std::vector<double>& {
+            if constexpr (std::is_same_v<T, double>)
+                return result.dose;
+            else
+                return dose;
+        }();
+        std::vector<double>& varianceOut = [&]() -> std::vector<double>& {
+            if constexpr (std::is_same_v<T, double>)
+                return result.variance;
+            else
+                return variance;
+        }();
         // the calibration factor (a nested transport for CT sources) is applied
         // by the device normalisation: dose = ei * (calibration / voxel mass)
         auto calibration = [&]() -> double {
@@ -186,19 +208,22 @@ public:
                 return static_cast<double>(source->getCalibrationValue(m_lowenergyCorrection, progressbar));
             return 1.0;
         };
-        const bool ok = runOnDevice(world, prep.density, prep.tables, prep.exposures, progressbar, calibration, dose, result.nEvents, variance,
-            result.numberOfHistories, context);
+        const bool ok = runOnDevice(world, prep, progressbar, calibration, doseOut, result.nEvents, varianceOut, result.numberOfHistories, context);
         result.simulationTime = std::chrono::steady_clock::now() - start;
         if (progressbar)
             progressbar->clearDoseData();
         if (!ok || (progressbar && progressbar->cancel())) {
+            std::fill(result.dose.begin(), result.dose.end(), T { 0 });
+            std::fill(result.variance.begin(), result.variance.end(), T { 0 });
             std::fill(result.nEvents.begin(), result.nEvents.end(), 0);
             result.numberOfHistories = 0;
             return result;
         }
-        for (std::size_t i = 0; i < dose.size(); ++i) {
-            result.dose[i] = static_cast<T>(dose[i]);
-            result.variance[i] = static_cast<T>(variance[i]);
+        if constexpr (!std::is_same_v<T, double>) {
+            for (std::size_t i = 0; i < dose.size(); ++i) {
+                result.dose[i] = static_cast<T>(dose[i]);
+                result.variance[i] = static_cast<T>(variance[i]);
+            }
         }
         if (m_outputmode == OUTPUTMODE::DOSE)
             result.dose_units = useSourceDoseCalibration ? "mGy" : "keV/kg";
@@ -223,9 +248,8 @@ public:
 
 protected:
     template <typename Calibration>
-    bool runOnDevice(const World<T>& world, const std::vector<double>& dens, const TableSet& tables, const std::vector<dxmc_exposure>& exposures,
-        ProgressBar<T>* pb, Calibration calibration, std::vector<double>& dose, std::vector<std::uint32_t>& nEvents, std::vector<double>& variance,
-        std::uint64_t nHist, dxmc_ctx* context)
+    bool runOnDevice(const World<T>& world, const Prepared& prep, ProgressBar<T>* pb, Calibration calibration, std::vector<double>& dose,
+        std::vector<std::uint32_t>& nEvents, std::vector<double>& variance, std::uint64_t nHist, dxmc_ctx* context)
     {
         dxmc_ctx* c = context ? context : DeviceContexts::get(m_device, m_error);
         if (!c)
@@ -238,7 +262,10 @@ protected:
         for (int i = 0; i < 6; ++i)
             wd.extent_safe[i] = static_cast<double>(world.matrixExtentSafe()[i]);
         wd.material = world.materialIndexArray()->data();
-        wd.density_f64 = dens.data();
+        if constexpr (std::is_same_v<T, double>)
+            wd.density_f64 = world.densityArray()->data();
+        else
+            wd.density_f32 = world.densityArray()->data();
         wd.measurement = world.measurementMapArray() ? world.measurementMapArray()->data() : nullptr;
         wd.n_materials = static_cast<std::uint32_t>(world.materialMap().size());
         dxmc_world* dw = nullptr;
@@ -252,18 +279,38 @@ protected:
             if (dw)
                 dxmc_hip_world_destroy(c, dw);
         };
-        const dxmc_tables_desc td = tables.desc();
         dxmc_run_cfg cfg = m_tuning;
         cfg.precision = m_precision >= 0 ? m_precision : (std::is_same_v<T, float> ? DXMC_PRECISION_FP32 : DXMC_PRECISION_FP64);
         cfg.low_energy_correction = static_cast<std::int32_t>(m_lowenergyCorrection);
         cfg.seed = m_seed;
         const int mode = m_outputmode == OUTPUTMODE::DOSE ? DXMC_OUTPUT_DOSE : DXMC_OUTPUT_EV_PER_HISTORY;
-        bool ok = !(dxmc_hip_world_upload(c, &wd, &dw) || dxmc_hip_tables_upload(c, &td, &dt) || dxmc_hip_tally_create(c, world.size(), &tally)
-            || dxmc_hip_run(c, dw, dt, exposures.data(), exposures.size(), &cfg, tally, pb ? pb->cancelFlag() : nullptr, pb ? pb->exposureCounter() : nullptr,
-                &m_stats));
-        if (ok && !(pb && pb->cancel()))
-            ok = !((nHist > 1 && dxmc_hip_tally_normalize(c, tally, dw, mode, nHist, mode == DXMC_OUTPUT_DOSE ? calibration() : 1.0))
-                || dxmc_hip_tally_download(c, tally, dose.data(), nEvents.data(), variance.data()));
+        // the majorant from the per-material maximum densities of the device scan
+        TableSet tables;
+        std::vector<double> maxDensity(wd.n_materials);
+        using clock = std::chrono::steady_clock;
+        const auto seconds = [](clock::time_point a, clock::time_point b) { return std::chrono::duration<double>(b - a).count(); };
+        const auto t0 = clock::now();
+        auto t1 = t0, t2 = t0;
+        bool ok = !(dxmc_hip_world_upload(c, &wd, &dw) || dxmc_hip_world_max_density(dw, maxDensity.data(), wd.n_materials));
+        if (ok) {
+            m_attenuationLut = prep.lut;
+            m_attenuationLut.setMaxDensity(maxDensity);
+            tables = prep.tables;
+            tables.majorant = m_attenuationLut.interpolator().maxCoefficients();
+            const dxmc_tables_desc td = tables.desc();
+            ok = !(dxmc_hip_tables_upload(c, &td, &dt) || dxmc_hip_tally_create(c, world.size(), &tally));
+            t1 = clock::now();
+            ok = ok
+                && !dxmc_hip_run(c, dw, dt, prep.exposures.data(), prep.exposures.size(), &cfg, tally, pb ? pb->cancelFlag() : nullptr,
+                    pb ? pb->exposureCounter() : nullptr, &m_stats);
+        }
+        if (ok && !(pb && pb->cancel())) {
+            ok = !(nHist > 1 && dxmc_hip_tally_normalize(c, tally, dw, mode, nHist, mode == DXMC_OUTPUT_DOSE ? calibration() : 1.0))
+                && !dxmc_hip_synchronize(c);
+            t2 = clock::now();
+            ok = ok && !dxmc_hip_tally_download(c, tally, dose.data(), nEvents.data(), variance.data());
+        }
+        m_timing = { seconds(t0, t1), seconds(t1, t2), seconds(t2, clock::now()) };
         if (!ok) {
             m_error = dxmc_hip_last_error(c);
             cleanup();
@@ -283,6 +330,7 @@ private:
     int m_precision = -1;
     dxmc_run_cfg m_tuning {};
     dxmc_run_stats m_stats {};
+    DeviceTiming m_timing {};
     std::string m_error;
 };
 }

@@ -28,6 +28,7 @@
 #include <cstdio>
 #include <cstring>
 #include <map>
+#include <memory>
 #include <mutex>
 #include <numeric>
 #include <stdexcept>
@@ -235,25 +236,62 @@ namespace {
         0.1074442701159656, 0.0976186521041139, 0.0861901615319533, 0.0733464814110803,
         0.0592985849154368, 0.0442774388174198, 0.0285313886289337, 0.0123412297999872 };
 
-    template <typename F>
-    double gaussLegendre(double a, double b, F f)
+    // Momentum-transfer panels of the scattering integrals: [0, 1e-4], then
+    // each panel four times longer than the one before, the last one cut at
+    // t_max = (E / hc)^2. Only that last panel depends on the energy, so the
+    // form factor and scatter function at the Gauss nodes of the full panels
+    // are evaluated once per element and reused; every node and every sum is
+    // formed by the same operations as with fresh evaluations, so the
+    // integrals are bit-identical. 16 panels reach t = 1e-4 * 4^16 ~ 4e5,
+    // beyond (2 m_e c^2 / hc)^2 ~ 6.8e3.
+    constexpr int FIXED_PANELS = 16;
+    struct PanelNodes {
+        double value[FIXED_PANELS][24];
+    };
+
+    template <typename V>
+    const PanelNodes& panelNodes(int kind, int Z, V value)
     {
-        const double c = 0.5 * (a + b), h = 0.5 * (b - a);
-        double s = 0;
-        for (int i = 0; i < 12; ++i)
-            s += GL_W[i] * (f(c - h * GL_X[i]) + f(c + h * GL_X[i]));
-        return s * h;
+        static std::map<std::pair<int, int>, std::unique_ptr<PanelNodes>> cache;
+        static std::mutex mutex;
+        std::lock_guard lock(mutex);
+        auto& nodes = cache[{ kind, Z }];
+        if (!nodes) {
+            nodes = std::make_unique<PanelNodes>();
+            double a = 0, b = 1e-4;
+            for (int j = 0; j < FIXED_PANELS; ++j) {
+                const double c = 0.5 * (a + b), h = 0.5 * (b - a);
+                for (int i = 0; i < 12; ++i) {
+                    nodes->value[j][2 * i] = value(c - h * GL_X[i]);
+                    nodes->value[j][2 * i + 1] = value(c + h * GL_X[i]);
+                }
+                a = b;
+                b *= 4;
+            }
+        }
+        return *nodes;
     }
 
-    // integrate g(t) over t = q^2 in [0, tmax] on log-spaced panels
-    template <typename F>
-    double integrateMomentum(double tmax, F g)
+    // integrate g(t, v(t)) over t = q^2 in [0, tmax] with 24-point
+    // Gauss-Legendre on each panel; v comes from the panel cache where the
+    // panel is whole
+    template <typename V, typename G>
+    double integrateMomentum(double tmax, const PanelNodes& nodes, V value, G g)
     {
         double sum = 0;
         double a = 0, b = 1e-4;
-        while (a < tmax) {
+        for (int j = 0; a < tmax; ++j) {
+            const bool whole = b <= tmax && j < FIXED_PANELS;
             b = std::min(b, tmax);
-            sum += gaussLegendre(a, b, g);
+            const double c = 0.5 * (a + b), h = 0.5 * (b - a);
+            double s = 0;
+            for (int i = 0; i < 12; ++i) {
+                const double t0 = c - h * GL_X[i], t1 = c + h * GL_X[i];
+                const double v0 = whole ? nodes.value[j][2 * i] : value(t0);
+                const double v1 = whole ? nodes.value[j][2 * i + 1] : value(t1);
+                s += GL_W[i] * (g(t0, v0) + g(t1, v1));
+            }
+            sum += s * h;
             a = b;
             b *= 4;
         }
@@ -264,9 +302,9 @@ namespace {
     {
         const double k = E / KEV_ANGSTROM;
         const double k2 = k * k;
-        const double integral = integrateMomentum(k2, [&](double t) {
+        const auto formFactor = [Z](double t) { return modelFormFactor(Z, std::sqrt(t)); };
+        const double integral = integrateMomentum(k2, panelNodes(0, Z, formFactor), formFactor, [&](double t, double F) {
             const double mu = 1 - 2 * t / k2;
-            const double F = modelFormFactor(Z, std::sqrt(t));
             return (1 + mu * mu) * F * F;
         });
         return 3.14159265358979323846 * RE2_BARN * 2 / k2 * integral;
@@ -283,9 +321,9 @@ namespace {
         const double k = E / KEV_ANGSTROM;
         const double k2 = k * k;
         const double kappa = E / MEC2;
-        const double integral = integrateMomentum(k2, [&](double t) {
+        const auto scatterFunction = [Z](double t) { return modelScatterFunction(Z, std::sqrt(t)); };
+        const double integral = integrateMomentum(k2, panelNodes(1, Z, scatterFunction), scatterFunction, [&](double t, double S) {
             const double mu = 1 - 2 * t / k2;
-            const double S = modelScatterFunction(Z, std::sqrt(t));
             const double kn = kleinNishinaFactor(mu, kappa);
             if (energyTransfer) {
                 const double r = 1 / (1 + kappa * (1 - mu));

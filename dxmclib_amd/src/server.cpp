@@ -59,17 +59,17 @@ struct Request {
     int32_t status = DXMC_OK;
     int32_t slot = -1;
     std::string error;
-    double tSubmit = 0, tPrepared = 0, tAdmitted = 0, tFinished = 0;
+    double tSubmit = 0, tPrepared = 0, tAdmitted = 0, tUploaded = 0, tTransported = 0, tFinished = 0;
 };
 
 // device memory a request holds while admitted: FP64 world (u8 material,
 // f64 density, optional u8 measurement), FP32 voxel records (at most 8 B per
-// voxel plus a palette), tally (f64 dose + variance, u32 count), tables and
-// exposure buffers
+// voxel plus a palette), tally (f64 dose + variance, u32 count), tables,
+// exposure buffers and the upload's temporaries (12 MB of palette hash)
 uint64_t deviceBytes(const Request& r)
 {
     const uint64_t n = r.world.dims[0] * r.world.dims[1] * r.world.dims[2];
-    return n * (1 + 8 + 8 + (r.measurement.empty() ? 0 : 1)) + 65536 * 8 + n * 20 + (8u << 20) + r.prep.exposures.size() * 512;
+    return n * (1 + 8 + 8 + (r.measurement.empty() ? 0 : 1)) + 65536 * 8 + n * 20 + (20u << 20) + r.prep.exposures.size() * 512;
 }
 
 }
@@ -107,8 +107,10 @@ struct dxmc_server {
             int status = DXMC_OK;
             if (!dxmc_host_detail::buildWorld(r->world, r->w, why)) {
                 status = DXMC_ERR_ARGUMENT;
+                r->error = why;
             } else if (!(r->source = dxmc_host_detail::makeSource(r->spec))) {
                 status = DXMC_ERR_ARGUMENT;
+                r->error = "unknown source type";
             } else {
                 r->transport = std::make_unique<Transport<double>>();
                 auto& t = *r->transport;
@@ -121,8 +123,10 @@ struct dxmc_server {
                 tuning.flags |= DXMC_RUN_COUNTERS;
                 t.setRunConfiguration(tuning);
                 r->prep = t.prepare(r->w, r->source.get());
-                if (!r->prep.valid)
+                if (!r->prep.valid) {
                     status = DXMC_ERR_ARGUMENT;
+                    r->error = "source is not valid for this world";
+                }
             }
             // the raw copies are no longer needed
             r->material = {};
@@ -132,8 +136,10 @@ struct dxmc_server {
             r->tPrepared = now();
             if (status == DXMC_OK) {
                 r->bytes = deviceBytes(*r);
-                if (r->bytes > budget)
+                if (r->bytes > budget) {
                     status = DXMC_ERR_ADMISSION;
+                    r->error = "needs " + std::to_string(r->bytes) + " bytes of device memory, more than the budget of " + std::to_string(budget);
+                }
             }
             if (status != DXMC_OK) {
                 r->status = status;
@@ -181,6 +187,9 @@ struct dxmc_server {
             const bool ok = r->result.numberOfHistories != 0;
             std::lock_guard lock(m);
             r->stats = r->transport->lastRunStats();
+            const auto& timing = r->transport->lastDeviceTiming();
+            r->tUploaded = r->tAdmitted + timing.upload;
+            r->tTransported = r->tUploaded + timing.transport;
             r->status = ok ? DXMC_OK : DXMC_ERR_DEVICE;
             r->error = r->transport->lastError();
             // keep the result only
@@ -344,6 +353,8 @@ int dxmc_host_server_collect(dxmc_server* s, uint64_t id, double* dose, uint32_t
     if (info) {
         info->submitted = r->tSubmit;
         info->prepared = r->tPrepared;
+        info->uploaded = r->tUploaded;
+        info->transported = r->tTransported;
         info->admitted = r->tAdmitted;
         info->finished = r->tFinished;
         info->histories = r->stats.histories;
@@ -354,7 +365,7 @@ int dxmc_host_server_collect(dxmc_server* s, uint64_t id, double* dose, uint32_t
         info->slot = r->slot;
     }
     if (r->status != DXMC_OK)
-        return serverFail(r->status, "request " + std::to_string(id) + ": " + (r->error.empty() ? "invalid world or source" : r->error));
+        return serverFail(r->status, "request " + std::to_string(id) + ": " + (r->error.empty() ? "failed" : r->error));
     {
         if (dose)
             std::copy(r->result.dose.cbegin(), r->result.dose.cend(), dose);

@@ -82,11 +82,19 @@ typedef struct {
     uint32_t n_materials;
 } dxmc_world_desc;
 
-/* Upload a world; the host arrays may be freed afterwards. */
+/* Upload a world; the host arrays may be freed afterwards. The passes over
+ * the voxels (material indices inside the material map, as World::validate
+ * checks them, world.hpp:182-227; per-material maximum density; the FP32
+ * voxel records) run on the device. */
 int dxmc_hip_world_upload(dxmc_ctx* ctx, const dxmc_world_desc* desc, dxmc_world** out);
 int dxmc_hip_world_destroy(dxmc_ctx* ctx, dxmc_world* world);
 /* Bytes the world occupies on the device. */
 int dxmc_hip_world_bytes(const dxmc_world* world, uint64_t* bytes);
+/* Largest density of each material over the world's voxels (0 for a material
+ * no voxel uses), computed on the device at upload: the scan of
+ * AttenuationLutInterpolator's majorant build (attenuationinterpolator.hpp
+ * :52-58). n must equal the world's n_materials. */
+int dxmc_hip_world_max_density(const dxmc_world* world, double* max_density, uint32_t n);
 
 /* ----------------------------------------------------------------- tables */
 /* One grouped electron shell: ElectronShellConfiguration (material.hpp:29-58). */

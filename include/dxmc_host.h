@@ -172,6 +172,10 @@ typedef struct {
     uint64_t device_bytes; /* memory charged against the budget */
     int32_t status; /* DXMC_OK or the error of the request */
     int32_t slot;
+    /* device phases on the server clock: world, tables and tally on the
+     * device (uploaded); transport and normalisation done (transported);
+     * then the download ends at 'finished' */
+    double uploaded, transported;
 } dxmc_request_info;
 
 int dxmc_host_server_create(const dxmc_server_cfg* cfg, dxmc_server** out);

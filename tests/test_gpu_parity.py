@@ -268,6 +268,58 @@ def test_fp32_voxel_records(device, records):
     assert abs(s32.events - s64.events) <= 1e-3 * s64.events
 
 
+@pytest.mark.parametrize("dtype", [np.float32, np.float64])
+def test_world_scan_on_device(device, dtype):
+    """The upload's passes over the voxels run on the device: each material's
+    maximum density (exact; reference attenuationinterpolator.hpp:52-58,
+    0 for an unused material), the material-index check of World::validate
+    (world.hpp:182-227) and whether any measurement flag is set."""
+    print(f"world scan {np.dtype(dtype).name}", flush=True)
+    rng = np.random.default_rng(11)
+    shape = (37, 41, 47)  # ragged sizes; more than 65536 distinct records
+    n = int(np.prod(shape))
+    mat = rng.integers(0, 4, shape).astype(np.uint8)  # material 4 of 5 unused
+    dens = rng.uniform(0.1, 2.5, shape).astype(dtype)
+    spacing = (1.0, 1.0, 1.0)
+    ext = backend.world_extent(shape[::-1], spacing)
+    want = np.array([dens[mat == m].max() if (mat == m).any() else 0.0 for m in range(5)], dtype=np.float64)
+
+    def upload(meas=None, n_materials=5):
+        return device.upload_world(mat, dens, spacing, ext, n_materials, measurement=meas)
+
+    w = upload()
+    try:
+        assert np.array_equal(device.world_max_density(w, 5), want)
+        assert device.world_bytes(w) == n * (1 + 8 + 8)
+    finally:
+        device.destroy_world(w)
+    # an all-zero measurement map is dropped; one flag keeps it
+    meas = np.zeros(shape, np.uint8)
+    w = upload(meas)
+    try:
+        assert device.world_bytes(w) == n * (1 + 8 + 8)
+    finally:
+        device.destroy_world(w)
+    meas[3, 4, 5] = 1
+    w = upload(meas)
+    try:
+        assert device.world_bytes(w) == n * (1 + 8 + 8 + 1)
+        assert np.array_equal(device.world_max_density(w, 5), want)
+    finally:
+        device.destroy_world(w)
+    # index 3 is outside a 3-material map
+    with pytest.raises(backend.DxmcError):
+        upload(n_materials=3)
+    # a uniform world: one palette record
+    one = np.full(shape, 2, np.uint8)
+    w = device.upload_world(one, np.full(shape, 1.5, dtype), spacing, ext, 3)
+    try:
+        assert device.world_bytes(w) == n * (1 + 8 + 1) + 8
+        assert np.array_equal(device.world_max_density(w, 3), [0.0, 0.0, 1.5])
+    finally:
+        device.destroy_world(w)
+
+
 def test_autotune_keeps_results(device):
     """The autotuner changes scheduling only: a run with the tuned
     configuration follows the same histories as one with the defaults."""

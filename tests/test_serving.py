@@ -80,18 +80,19 @@ def test_same_alone_and_in_mix():
 @pytest.mark.gpu
 def test_admission_by_memory():
     reqs = [_request(40, 100.0, 300_000, True, 31), _request(40, 120.0, 300_000, False, 32), _request(40, 80.0, 300_000, True, 33)]
-    n = 40 ** 3
-    one = n * (1 + 8 + 8 + 20) + 65536 * 8 + (8 << 20) + 360 * 512
+    with backend.Server(slots=4) as srv:
+        free = _run_all(srv, reqs, [1, 2, 3])
+    # the memory each request is charged; any two exceed the budget below
+    one = max(r["info"].device_bytes for r in free)
+    assert 2 * min(r["info"].device_bytes for r in free) > 1.5 * one
     with backend.Server(slots=4, memory_budget=int(one * 1.5)) as srv:
         small = _run_all(srv, reqs, [1, 2, 3])
         # never more than one admitted at a time under this budget
         spans = sorted((r["info"].admitted, r["info"].finished) for r in small)
         assert all(s1 >= e0 for (s0, e0), (s1, e1) in zip(spans, spans[1:]))
-        big_world, big_spec = _request(96, 100.0, 1000, True, 34)
+        big_world, big_spec = _request(128, 100.0, 1000, True, 34)
         with pytest.raises(backend.DxmcError):
             srv.submit(big_world, big_spec)
-    with backend.Server(slots=4) as srv:
-        free = _run_all(srv, reqs, [1, 2, 3])
     for x, y in zip(small, free):
         assert x["info"].steps == y["info"].steps and x["info"].events == y["info"].events
         assert np.allclose(x["dose"], y["dose"], rtol=1e-10, atol=0)
