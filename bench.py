@@ -168,14 +168,17 @@ def serving(n_requests, rate, slots, prep_threads):
     run = np.array([x.finished - x.admitted for x in infos])
     upload = np.array([x.uploaded - x.admitted for x in infos])
     transport = np.array([x.transported - x.uploaded for x in infos])
-    download = np.array([x.finished - x.transported for x in infos])
+    download = np.array([x.downloaded - x.transported for x in infos])
+    release = np.array([x.released - x.downloaded for x in infos])
+    publish = np.array([x.finished - x.released for x in infos])
     hist = sum(x.histories for x in infos)
     span = max(x.finished for x in infos) - min(x.submitted for x in infos)
     return {"requests": len(infos), "rate_per_s": rate, "slots": slots, "prep_threads": prep_threads,
             "latency_p50_s": float(np.percentile(lat, 50)), "latency_p99_s": float(np.percentile(lat, 99)),
             "latency_mean_s": float(lat.mean()), "prepare_mean_s": float(prep.mean()), "admission_wait_mean_s": float(wait.mean()),
             "device_mean_s": float(run.mean()), "upload_mean_s": float(upload.mean()), "transport_mean_s": float(transport.mean()),
-            "download_mean_s": float(download.mean()), "histories": int(hist), "makespan_s": span, "histories_per_s": hist / span,
+            "download_mean_s": float(download.mean()), "release_mean_s": float(release.mean()),
+            "publish_mean_s": float(publish.mean()), "histories": int(hist), "makespan_s": span, "histories_per_s": hist / span,
             "failed": int(sum(1 for x in infos if x.status != 0))}
 
 

@@ -256,6 +256,8 @@ class RequestInfo(C.Structure):
         ("slot", C.c_int32),
         ("uploaded", C.c_double),
         ("transported", C.c_double),
+        ("downloaded", C.c_double),
+        ("released", C.c_double),
     ]
 
 

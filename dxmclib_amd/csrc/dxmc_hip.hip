@@ -284,6 +284,7 @@ int dxmc_hip_ctx_destroy(dxmc_ctx* c)
     hipStreamSynchronize(c->stream);
     hipEventDestroy(c->ev0);
     hipEventDestroy(c->ev1);
+    c->releaseStaging();
     hipFree(c->historyCounter);
     hipStreamDestroy(c->stream);
     delete c;

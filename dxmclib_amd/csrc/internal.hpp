@@ -5,6 +5,8 @@
 #include "dxmc_hip.h"
 
 #include <hip/hip_runtime.h>
+#include <algorithm>
+#include <cstring>
 #include <map>
 #include <string>
 #include <tuple>
@@ -58,13 +60,64 @@ struct dxmc_ctx {
         if (p)
             hipFreeAsync(p, stream);
     }
+    // Large host transfers go through two pinned chunks, so the DMA of one
+    // chunk overlaps the host copy of the other; from pageable memory the
+    // runtime's own staging reaches a fraction of the link rate. The chunks
+    // are allocated at the context's first large transfer.
+    static constexpr size_t STAGE_CHUNK = size_t { 8 } << 20;
+    static constexpr size_t STAGE_MIN = size_t { 1 } << 20;
+    void* stage[2] = { nullptr, nullptr };
+    hipEvent_t stageDone[2] = { nullptr, nullptr };
+
+    bool staging()
+    {
+        if (stage[0])
+            return true;
+        for (int i = 0; i < 2; ++i)
+            if (hipHostMalloc(&stage[i], STAGE_CHUNK, hipHostMallocDefault) != hipSuccess
+                || hipEventCreateWithFlags(&stageDone[i], hipEventDisableTiming) != hipSuccess) {
+                hipGetLastError();
+                releaseStaging();
+                return false;
+            }
+        return true;
+    }
+    void releaseStaging()
+    {
+        for (int i = 0; i < 2; ++i) {
+            if (stage[i])
+                hipHostFree(stage[i]);
+            if (stageDone[i])
+                hipEventDestroy(stageDone[i]);
+            stage[i] = nullptr;
+            stageDone[i] = nullptr;
+        }
+    }
     // host -> device, complete on return (the host array may be freed)
     int copy(void* dst, const void* src, size_t bytes)
     {
         if (bytes == 0)
             return DXMC_OK;
-        if (int st = check(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, stream)))
-            return st;
+        if (bytes < STAGE_MIN || !staging()) {
+            if (int st = check(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, stream)))
+                return st;
+            return check(hipStreamSynchronize(stream));
+        }
+        const auto* from = static_cast<const char*>(src);
+        auto* to = static_cast<char*>(dst);
+        for (size_t off = 0, i = 0; off < bytes; off += STAGE_CHUNK, ++i) {
+            const int b = static_cast<int>(i & 1);
+            const size_t len = std::min(STAGE_CHUNK, bytes - off);
+            // the chunk's previous DMA (two chunks back) has to be done
+            if (i >= 2)
+                if (int st = check(hipEventSynchronize(stageDone[b])))
+                    return st;
+            std::memcpy(stage[b], from + off, len);
+            if (int st = check(hipMemcpyAsync(to + off, stage[b], len, hipMemcpyHostToDevice, stream)))
+                return st;
+            if (int st = check(hipEventRecord(stageDone[b], stream)))
+                return st;
+        }
         return check(hipStreamSynchronize(stream));
     }
     // device -> host, complete on return
@@ -72,9 +125,40 @@ struct dxmc_ctx {
     {
         if (bytes == 0)
             return DXMC_OK;
-        if (int st = check(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, stream)))
-            return st;
-        return check(hipStreamSynchronize(stream));
+        if (bytes < STAGE_MIN || !staging()) {
+            if (int st = check(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, stream)))
+                return st;
+            return check(hipStreamSynchronize(stream));
+        }
+        const auto* from = static_cast<const char*>(src);
+        auto* to = static_cast<char*>(dst);
+        size_t pendingOff[2] = { 0, 0 }, pendingLen[2] = { 0, 0 };
+        auto drain = [&](int b) {
+            if (int st = check(hipEventSynchronize(stageDone[b])))
+                return st;
+            std::memcpy(to + pendingOff[b], stage[b], pendingLen[b]);
+            pendingLen[b] = 0;
+            return 0;
+        };
+        for (size_t off = 0, i = 0; off < bytes; off += STAGE_CHUNK, ++i) {
+            const int b = static_cast<int>(i & 1);
+            const size_t len = std::min(STAGE_CHUNK, bytes - off);
+            if (int st = check(hipMemcpyAsync(stage[b], from + off, len, hipMemcpyDeviceToHost, stream)))
+                return st;
+            if (int st = check(hipEventRecord(stageDone[b], stream)))
+                return st;
+            pendingOff[b] = off;
+            pendingLen[b] = len;
+            // copy out the previous chunk while this one is in flight
+            if (pendingLen[1 - b])
+                if (int st = drain(1 - b))
+                    return st;
+        }
+        for (int b = 0; b < 2; ++b)
+            if (pendingLen[b])
+                if (int st = drain(b))
+                    return st;
+        return DXMC_OK;
     }
 };
 

@@ -117,9 +117,10 @@ public:
     void setRunConfiguration(const dxmc_run_cfg& cfg) { m_tuning = cfg; }
     const dxmc_run_stats& lastRunStats() const { return m_stats; }
     // wall time of the device phases of the last run [s]: world, tables and
-    // tally to the device; transport and normalisation; tally to the host
+    // tally to the device; transport and normalisation; tally to the host;
+    // release of the device objects
     struct DeviceTiming {
-        double upload = 0, transport = 0, download = 0;
+        double upload = 0, transport = 0, download = 0, release = 0;
     };
     const DeviceTiming& lastDeviceTiming() const { return m_timing; }
     const std::string& lastError() const { return m_error; }
@@ -177,46 +178,30 @@ public:
         m_error.clear();
         if (!prep.valid)
             return result;
-        result.numberOfHistories = prep.histories;
-        if (progressbar) {
-            progressbar->setTotalExposures(prep.exposures.size());
+        if (progressbar)
             progressbar->setDoseData(result.dose.data(), world.dimensions(), world.spacing());
-        }
         const auto start = std::chrono::steady_clock::now();
         // FP64 tallies go straight into the result when T is double
         std::vector<double> dose, variance;
-        if constexpr (!std::is_same_v<T, double>) {
+        double* doseOut = nullptr;
+        double* varianceOut = nullptr;
+        if constexpr (std::is_same_v<T, double>) {
+            doseOut = result.dose.data();
+            varianceOut = result.variance.data();
+        } else {
             dose.resize(world.size());
             variance.resize(world.size());
+            doseOut = dose.data();
+            varianceOut = variance.data();
         }
-        std::vector<double>& doseOut = [&]() -> std::vector<double>& {
-            if constexpr (std::is_same_v<T, double>)
-                return result.dose;
-            else
-                return dose;
-        }();
-        std::vector<double>& varianceOut = [&]() -> std::vector<double>& {
-            if constexpr (std::is_same_v<T, double>)
-                return result.variance;
-            else
-                return variance;
-        }();
-        // the calibration factor (a nested transport for CT sources) is applied
-        // by the device normalisation: dose = ei * (calibration / voxel mass)
-        auto calibration = [&]() -> double {
-            if (m_outputmode == OUTPUTMODE::DOSE && useSourceDoseCalibration)
-                return static_cast<double>(source->getCalibrationValue(m_lowenergyCorrection, progressbar));
-            return 1.0;
-        };
-        const bool ok = runOnDevice(world, prep, progressbar, calibration, doseOut, result.nEvents, varianceOut, result.numberOfHistories, context);
+        result.numberOfHistories = executeInto(world, source, prep, doseOut, result.nEvents.data(), varianceOut, progressbar, useSourceDoseCalibration, context);
         result.simulationTime = std::chrono::steady_clock::now() - start;
         if (progressbar)
             progressbar->clearDoseData();
-        if (!ok || (progressbar && progressbar->cancel())) {
+        if (result.numberOfHistories == 0) {
             std::fill(result.dose.begin(), result.dose.end(), T { 0 });
             std::fill(result.variance.begin(), result.variance.end(), T { 0 });
             std::fill(result.nEvents.begin(), result.nEvents.end(), 0);
-            result.numberOfHistories = 0;
             return result;
         }
         if constexpr (!std::is_same_v<T, double>) {
@@ -230,6 +215,33 @@ public:
         else
             result.dose_units = "eV/history";
         return result;
+    }
+
+    // execute() into caller-owned host arrays of world.size() elements: FP64
+    // dose (or energy imparted) and variance, and event counts. Returns the
+    // number of histories, or 0 on failure or cancel (the arrays are then
+    // unspecified). Lets a caller that runs many requests reuse its arrays.
+    template <typename U>
+        requires std::is_base_of<World<T>, U>::value
+    std::uint64_t executeInto(const U& world, Source<T>* source, const Prepared& prep, double* dose, std::uint32_t* nEvents, double* variance,
+        ProgressBar<T>* progressbar = nullptr, bool useSourceDoseCalibration = true, dxmc_ctx* context = nullptr)
+    {
+        m_error.clear();
+        if (!prep.valid || !dose || !nEvents || !variance)
+            return 0;
+        if (progressbar)
+            progressbar->setTotalExposures(prep.exposures.size());
+        // the calibration factor (a nested transport for CT sources) is applied
+        // by the device normalisation: dose = ei * (calibration / voxel mass)
+        auto calibration = [&]() -> double {
+            if (m_outputmode == OUTPUTMODE::DOSE && useSourceDoseCalibration)
+                return static_cast<double>(source->getCalibrationValue(m_lowenergyCorrection, progressbar));
+            return 1.0;
+        };
+        const bool ok = runOnDevice(world, prep, progressbar, calibration, dose, nEvents, variance, prep.histories, context);
+        if (!ok || (progressbar && progressbar->cancel()))
+            return 0;
+        return prep.histories;
     }
 
     // Transport<T>::operator() of the reference (transport.hpp:135-202)
@@ -248,8 +260,8 @@ public:
 
 protected:
     template <typename Calibration>
-    bool runOnDevice(const World<T>& world, const Prepared& prep, ProgressBar<T>* pb, Calibration calibration, std::vector<double>& dose,
-        std::vector<std::uint32_t>& nEvents, std::vector<double>& variance, std::uint64_t nHist, dxmc_ctx* context)
+    bool runOnDevice(const World<T>& world, const Prepared& prep, ProgressBar<T>* pb, Calibration calibration, double* dose, std::uint32_t* nEvents,
+        double* variance, std::uint64_t nHist, dxmc_ctx* context)
     {
         dxmc_ctx* c = context ? context : DeviceContexts::get(m_device, m_error);
         if (!c)
@@ -308,16 +320,14 @@ protected:
             ok = !(nHist > 1 && dxmc_hip_tally_normalize(c, tally, dw, mode, nHist, mode == DXMC_OUTPUT_DOSE ? calibration() : 1.0))
                 && !dxmc_hip_synchronize(c);
             t2 = clock::now();
-            ok = ok && !dxmc_hip_tally_download(c, tally, dose.data(), nEvents.data(), variance.data());
+            ok = ok && !dxmc_hip_tally_download(c, tally, dose, nEvents, variance);
         }
-        m_timing = { seconds(t0, t1), seconds(t1, t2), seconds(t2, clock::now()) };
-        if (!ok) {
+        const auto t3 = clock::now();
+        if (!ok)
             m_error = dxmc_hip_last_error(c);
-            cleanup();
-            return false;
-        }
         cleanup();
-        return true;
+        m_timing = { seconds(t0, t1), seconds(t1, t2), seconds(t2, t3), seconds(t3, clock::now()) };
+        return ok;
     }
 
 private:

@@ -194,7 +194,7 @@ std::unique_ptr<Source<double>> makeSource(const dxmc_source_spec& s)
     }
 }
 
-bool buildWorld(const dxmc_host_world& w, World<double>& world, std::string& why)
+bool buildWorld(const dxmc_host_world& w, World<double>& world, std::string& why, const WorldArrays* arrays)
 {
     if (!w.material || !w.density || !w.material_names || w.n_materials == 0)
         return why = "world arrays or material names missing", false;
@@ -203,12 +203,20 @@ bool buildWorld(const dxmc_host_world& w, World<double>& world, std::string& why
     world.setOrigin({ w.origin[0], w.origin[1], w.origin[2] });
     world.setDirectionCosines(cosinesOrIdentity(w.direction_cosines));
     const std::size_t n = w.dims[0] * w.dims[1] * w.dims[2];
-    auto dens = std::make_shared<std::vector<double>>(w.density, w.density + n);
-    auto mat = std::make_shared<std::vector<std::uint8_t>>(w.material, w.material + n);
+    auto dens = arrays ? arrays->density(n) : std::make_shared<std::vector<double>>();
+    auto mat = arrays ? arrays->bytes(n) : std::make_shared<std::vector<std::uint8_t>>();
+    dens->assign(w.density, w.density + n);
+    mat->assign(w.material, w.material + n);
     world.setDensityArray(dens);
     world.setMaterialIndexArray(mat);
-    if (w.measurement)
-        world.setMeasurementMapArray(std::make_shared<std::vector<std::uint8_t>>(w.measurement, w.measurement + n));
+    if (w.measurement || arrays) {
+        auto meas = arrays ? arrays->bytes(n) : std::make_shared<std::vector<std::uint8_t>>();
+        if (w.measurement)
+            meas->assign(w.measurement, w.measurement + n);
+        else
+            meas->assign(n, 0);
+        world.setMeasurementMapArray(meas);
+    }
     for (uint32_t i = 0; i < w.n_materials; ++i) {
         Material m(w.material_names[i]);
         if (!m.isValid())

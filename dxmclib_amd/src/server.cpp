@@ -16,6 +16,7 @@
 // what else runs.
 #include "dxmc_host.h"
 #include "host_internal.hpp"
+#include "hostpool.hpp"
 
 #include "dxmc/transport.hpp"
 
@@ -31,6 +32,7 @@
 #include <vector>
 
 using namespace dxmc;
+using dxmc_host_detail::VectorPool;
 
 namespace {
 
@@ -38,8 +40,8 @@ struct Request {
     uint64_t id = 0;
     // inputs, owned
     dxmc_host_world world {};
-    std::vector<uint8_t> material, measurement;
-    std::vector<float> density;
+    VectorPool<uint8_t>::Handle material, measurement;
+    VectorPool<float>::Handle density;
     std::vector<std::string> names;
     std::vector<const char*> namePtrs;
     dxmc_source_spec spec {};
@@ -52,14 +54,19 @@ struct Request {
     std::unique_ptr<Transport<double>> transport;
     Transport<double>::Prepared prep;
     uint64_t bytes = 0;
-    // result
-    Result<double> result;
+    // result: n values each of dose (or energy imparted) and variance
+    // (f64), then event counts (u32), in one block
+    uint64_t n = 0;
+    dxmc_host_detail::BlockPool::Handle result;
+    double* dose() const { return reinterpret_cast<double*>(result.get()); }
+    double* variance() const { return dose() + n; }
+    uint32_t* events() const { return reinterpret_cast<uint32_t*>(variance() + n); }
     dxmc_run_stats stats {};
     int32_t state = DXMC_REQUEST_QUEUED;
     int32_t status = DXMC_OK;
     int32_t slot = -1;
     std::string error;
-    double tSubmit = 0, tPrepared = 0, tAdmitted = 0, tUploaded = 0, tTransported = 0, tFinished = 0;
+    double tSubmit = 0, tPrepared = 0, tAdmitted = 0, tUploaded = 0, tTransported = 0, tDownloaded = 0, tReleased = 0, tFinished = 0;
 };
 
 // device memory a request holds while admitted: FP64 world (u8 material,
@@ -69,13 +76,20 @@ struct Request {
 uint64_t deviceBytes(const Request& r)
 {
     const uint64_t n = r.world.dims[0] * r.world.dims[1] * r.world.dims[2];
-    return n * (1 + 8 + 8 + (r.measurement.empty() ? 0 : 1)) + 65536 * 8 + n * 20 + (20u << 20) + r.prep.exposures.size() * 512;
+    return n * (1 + 8 + 8 + (r.world.measurement ? 1 : 0)) + 65536 * 8 + n * 20 + (20u << 20) + r.prep.exposures.size() * 512;
 }
 
 }
 
 struct dxmc_server {
     dxmc_server_cfg cfg {};
+    // host arrays reused across requests (declared first: they outlive the
+    // requests that hold them)
+    VectorPool<uint8_t> bytes { size_t { 2 } << 30 };
+    VectorPool<float> floats { size_t { 4 } << 30 };
+    VectorPool<double> doubles { size_t { 8 } << 30 };
+    dxmc_host_detail::BlockPool results { size_t { 8 } << 30 };
+    dxmc_host_detail::WorldArrays worldArrays { [this](size_t n) { return doubles.take(n); }, [this](size_t n) { return bytes.take(n); } };
     std::chrono::steady_clock::time_point t0 = std::chrono::steady_clock::now();
     std::mutex m;
     std::condition_variable cvPrep, cvReady, cvDone;
@@ -105,7 +119,7 @@ struct dxmc_server {
             }
             std::string why;
             int status = DXMC_OK;
-            if (!dxmc_host_detail::buildWorld(r->world, r->w, why)) {
+            if (!dxmc_host_detail::buildWorld(r->world, r->w, why, &worldArrays)) {
                 status = DXMC_ERR_ARGUMENT;
                 r->error = why;
             } else if (!(r->source = dxmc_host_detail::makeSource(r->spec))) {
@@ -129,9 +143,9 @@ struct dxmc_server {
                 }
             }
             // the raw copies are no longer needed
-            r->material = {};
-            r->density = {};
-            r->measurement = {};
+            r->material.reset();
+            r->density.reset();
+            r->measurement.reset();
             std::lock_guard lock(m);
             r->tPrepared = now();
             if (status == DXMC_OK) {
@@ -183,19 +197,25 @@ struct dxmc_server {
                 r->slot = slot;
                 r->tAdmitted = now();
             }
-            r->result = r->transport->execute(r->w, r->source.get(), r->prep, nullptr, false, ctx);
-            const bool ok = r->result.numberOfHistories != 0;
+            r->n = r->w.size();
+            r->result = results.take(r->n * (8 + 8 + 4));
+            const double tStart = now();
+            const bool ok = r->transport->executeInto(r->w, r->source.get(), r->prep, r->dose(), r->events(), r->variance(), nullptr, false, ctx) != 0;
             std::lock_guard lock(m);
             r->stats = r->transport->lastRunStats();
             const auto& timing = r->transport->lastDeviceTiming();
-            r->tUploaded = r->tAdmitted + timing.upload;
+            r->tUploaded = tStart + timing.upload;
             r->tTransported = r->tUploaded + timing.transport;
+            r->tDownloaded = r->tTransported + timing.download;
+            r->tReleased = r->tDownloaded + timing.release;
             r->status = ok ? DXMC_OK : DXMC_ERR_DEVICE;
             r->error = r->transport->lastError();
             // keep the result only
             r->prep = {};
             r->w = World<double>();
             r->source.reset();
+            if (!ok)
+                r->result.reset();
             r->tFinished = now();
             r->state = DXMC_REQUEST_DONE;
             inUse -= r->bytes;
@@ -271,18 +291,22 @@ int dxmc_host_server_submit(dxmc_server* s, const dxmc_host_world* w, const dxmc
         return serverFail(DXMC_ERR_ARGUMENT, "server_submit: world arrays or material names missing");
     auto r = std::make_shared<Request>();
     const uint64_t n = w->dims[0] * w->dims[1] * w->dims[2];
-    r->material.assign(w->material, w->material + n);
-    r->density.assign(w->density, w->density + n);
-    if (w->measurement)
-        r->measurement.assign(w->measurement, w->measurement + n);
+    r->material = s->bytes.take(n);
+    r->material->assign(w->material, w->material + n);
+    r->density = s->floats.take(n);
+    r->density->assign(w->density, w->density + n);
+    if (w->measurement) {
+        r->measurement = s->bytes.take(n);
+        r->measurement->assign(w->measurement, w->measurement + n);
+    }
     for (uint32_t i = 0; i < w->n_materials; ++i)
         r->names.emplace_back(w->material_names[i] ? w->material_names[i] : "");
     for (const auto& nm : r->names)
         r->namePtrs.push_back(nm.c_str());
     r->world = *w;
-    r->world.material = r->material.data();
-    r->world.density = r->density.data();
-    r->world.measurement = w->measurement ? r->measurement.data() : nullptr;
+    r->world.material = r->material->data();
+    r->world.density = r->density->data();
+    r->world.measurement = w->measurement ? r->measurement->data() : nullptr;
     r->world.material_names = r->namePtrs.data();
     r->spec = *spec;
     if (spec->bowtie_n && spec->bowtie_angles && spec->bowtie_weights) {
@@ -355,6 +379,8 @@ int dxmc_host_server_collect(dxmc_server* s, uint64_t id, double* dose, uint32_t
         info->prepared = r->tPrepared;
         info->uploaded = r->tUploaded;
         info->transported = r->tTransported;
+        info->downloaded = r->tDownloaded;
+        info->released = r->tReleased;
         info->admitted = r->tAdmitted;
         info->finished = r->tFinished;
         info->histories = r->stats.histories;
@@ -368,11 +394,11 @@ int dxmc_host_server_collect(dxmc_server* s, uint64_t id, double* dose, uint32_t
         return serverFail(r->status, "request " + std::to_string(id) + ": " + (r->error.empty() ? "failed" : r->error));
     {
         if (dose)
-            std::copy(r->result.dose.cbegin(), r->result.dose.cend(), dose);
+            std::copy_n(r->dose(), r->n, dose);
         if (nEvents)
-            std::copy(r->result.nEvents.cbegin(), r->result.nEvents.cend(), nEvents);
+            std::copy_n(r->events(), r->n, nEvents);
         if (variance)
-            std::copy(r->result.variance.cbegin(), r->result.variance.cend(), variance);
+            std::copy_n(r->variance(), r->n, variance);
     }
     return DXMC_OK;
 }

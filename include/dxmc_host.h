@@ -172,10 +172,12 @@ typedef struct {
     uint64_t device_bytes; /* memory charged against the budget */
     int32_t status; /* DXMC_OK or the error of the request */
     int32_t slot;
-    /* device phases on the server clock: world, tables and tally on the
-     * device (uploaded); transport and normalisation done (transported);
-     * then the download ends at 'finished' */
-    double uploaded, transported;
+    /* device phases on the server clock: host result arrays taken, world,
+     * tables and tally on the device (uploaded); transport and
+     * normalisation done (transported); tally on the host (downloaded);
+     * device objects released (released); host objects released and the
+     * result published at 'finished' */
+    double uploaded, transported, downloaded, released;
 } dxmc_request_info;
 
 int dxmc_host_server_create(const dxmc_server_cfg* cfg, dxmc_server** out);

@@ -48,6 +48,23 @@ def test_rng_streams_bit_exact(device, seed, first):
     assert np.array_equal(gpu, cpu)
 
 
+def test_large_transfers(device, dev_tables, orc):
+    """Host transfers above 1 MiB go through two pinned 8 MiB chunks; sizes
+    with a partial last chunk come back intact in both directions."""
+    n = (1 << 20) + 7  # x 4 draws x 4 B = 16 MiB + 112 B
+    print(f"large transfers: rng streams n={n}", flush=True)
+    assert np.array_equal(device.rng_streams(42, 12345, n, 4), oracle.rng_streams(42, 12345, n, 4))
+    rng = np.random.default_rng(9)
+    m = 1_200_001  # energies 9.2 MiB up, attenuation 27 MiB down
+    mat = rng.integers(0, len(NAMES), m)
+    e = 10 ** rng.uniform(0.0, np.log10(120.0), m)
+    print(f"large transfers: LUT lookups n={m}", flush=True)
+    ga, gm = device.lut_lookup(dev_tables, capi.PRECISION_FP64, mat, e)
+    ca, cm = orc.lut_lookup(0, mat, e)
+    assert np.max(np.abs(ga - ca) / ca) <= 1e-12
+    assert np.max(np.abs(gm - cm) / cm) <= 1e-12
+
+
 def _energies(n, rng):
     e = 10 ** rng.uniform(0.0, np.log10(120.0), n)
     e[:40] = np.array([1.0, 1.07, 1.0708, 1.305, 2.146, 2.4719, 2.472, 4.0381, 4.04, 30.0] * 4)  # edges and grid points
