@@ -1,19 +1,12 @@
-// X-ray tube: energy grid, spectrum, filtration and half value layer.
-// Interface of the reference (tube.hpp:35-299). The spectrum itself is this
-// project's own thick-target model (the reference's semi-relativistic
-// Bethe-Heitler integral over tabulated electron depth distributions is not
-// reproduced):
-//   N(hv) = 1/hv * integral_{hv}^{T0} exp(-mu_W(hv) x(T) / sin(takeoff)) dT
-// i.e. Kramers' constant energy-loss emission with the electron energy
-// falling with depth by the Thomson-Whiddington law
-//   x(T) = R(T0) (1 - T^2/T0^2),  R(T0) = 0.0119 T0^1.513 mg/cm2 (tungsten)
-// plus the four tungsten K lines (59.3, 58.0, 67.2, 69.1 keV; relative
-// intensities 0.505, 0.291, 0.162, 0.042) with a (T0/E_K - 1)^1.63 yield,
-// attenuated by the anode at the mean production depth, then filtered by
-// the added filtration. Spectrum and heel tables are inputs to the
-// transport; GPU and oracle sample identical tables.
+// X-ray tube: energy grid, spectrum, filtration and half value layer
+// (reference tube.hpp:35-299). The spectrum is the reference's model:
+// thick-target Bethe-Heitler bremsstrahlung of a tungsten anode over
+// tabulated electron depth distributions, plus the tungsten K lines, each
+// attenuated in the anode along the take-off direction (anodespectrum.hpp),
+// then attenuated by the added filtration and normalised to unit sum.
 #pragma once
 
+#include "dxmc/anodespectrum.hpp"
 #include "dxmc/constants.hpp"
 #include "dxmc/floating.hpp"
 #include "dxmc/material.hpp"
@@ -108,21 +101,23 @@ public:
     std::vector<T> getSpecter(const std::vector<T>& energies, bool normalize = true) const { return getSpecter(energies, m_anodeAngle, normalize); }
     std::vector<T> getSpecter(const std::vector<T>& energies, const T takeoffAngle, bool normalize = true) const
     {
+        auto& anodeModel = anode::SpectrumCache::instance();
+        const double T0 = static_cast<double>(m_voltage);
         std::vector<T> s(energies.size());
         for (std::size_t i = 0; i < energies.size(); ++i)
-            s[i] = static_cast<T>(bremsstrahlung(m_voltage, energies[i], takeoffAngle));
+            s[i] = static_cast<T>(anodeModel.bremsstrahlung(T0, static_cast<double>(energies[i]), static_cast<double>(takeoffAngle)));
         addCharacteristic(energies, s, takeoffAngle);
         for (const auto& [mat, mm] : m_filtrationMaterials) {
-            const double cm = mm * 0.1;
+            const T cm = mm * T { 0.1 };
             for (std::size_t i = 0; i < energies.size(); ++i)
-                s[i] *= static_cast<T>(std::exp(-mat.getTotalAttenuation(energies[i]) * mat.standardDensity() * cm));
+                s[i] = s[i] * static_cast<T>(std::exp(-mat.getTotalAttenuation(energies[i]) * mat.standardDensity() * cm));
         }
         if (normalize) {
             T sum = 0;
             for (const auto v : s)
-                sum += v;
+                sum = sum + v;
             for (auto& v : s)
-                v /= sum;
+                v = v / sum;
         }
         return s;
     }
@@ -138,44 +133,17 @@ public:
     T mmAlHalfValueLayer() const { return m_hasCachedHVL ? m_cachedHVL : computeHVL(); }
 
 protected:
-    static double anodeRange(double T0) { return 0.0119 * std::pow(T0, 1.513); } // mg/cm2
-
-    static double bremsstrahlung(double T0, double hv, double takeoff)
-    {
-        if (hv <= 0 || hv >= T0)
-            return 0;
-        const double mu = Material::getTotalAttenuation(74, hv) * 1e-3 / std::sin(takeoff); // per mg/cm2
-        const double R = anodeRange(T0);
-        // integral over T in [hv, T0] by 64-point midpoint rule
-        constexpr int n = 64;
-        const double dT = (T0 - hv) / n;
-        double s = 0;
-        for (int i = 0; i < n; ++i) {
-            const double t = hv + (i + 0.5) * dT;
-            const double x = R * (1 - (t * t) / (T0 * T0));
-            s += std::exp(-mu * x);
-        }
-        return s * dT / hv;
-    }
-
+    // each K line goes to the first energy bin at or above it, if that bin
+    // is within 2 keV (reference tube.hpp:216-236)
     void addCharacteristic(const std::vector<T>& energies, std::vector<T>& s, T takeoff) const
     {
-        constexpr double kEdge = 69.525;
-        if (m_voltage <= kEdge || energies.empty())
-            return;
-        constexpr double lineE[4] = { 59.3, 58.0, 67.2, 69.1 };
-        constexpr double lineF[4] = { 0.505, 0.291, 0.162, 0.042 };
-        double brems = 0;
-        for (const auto e : energies)
-            brems += bremsstrahlung(m_voltage, e, takeoff);
-        const double yield = 0.1 * std::pow(m_voltage / kEdge - 1, 1.63) * brems;
-        const double depth = anodeRange(m_voltage) / 3;
-        for (int i = 0; i < 4; ++i) {
-            const auto it = std::lower_bound(energies.cbegin(), energies.cend(), static_cast<T>(lineE[i]));
-            if (it == energies.cend() || std::abs(lineE[i] - *it) > 2.0)
+        for (const auto& [lineEnergy, fraction] : anode::K_LINES) {
+            const T e = static_cast<T>(lineEnergy);
+            const auto it = std::lower_bound(energies.cbegin(), energies.cend(), e);
+            if (it == energies.cend() || std::abs(e - *it) > T { 2 })
                 continue;
-            const double mu = Material::getTotalAttenuation(74, lineE[i]) * 1e-3 / std::sin(takeoff);
-            s[it - energies.cbegin()] += static_cast<T>(lineF[i] * yield * std::exp(-mu * depth));
+            const auto i = static_cast<std::size_t>(it - energies.cbegin());
+            s[i] = s[i] + static_cast<T>(anode::kLineIntensity(static_cast<double>(m_voltage), lineEnergy, fraction, static_cast<double>(takeoff)));
         }
     }
 

@@ -78,7 +78,33 @@ def lib():
         L.oracle_transport.argtypes = [C.c_void_p, C.POINTER(WorldView), C.c_void_p, C.c_uint64, C.c_int, C.c_int, C.c_uint64, C.c_uint64,
                                        C.c_uint32, f64p, u32p, f64p, u64p]
         L.oracle_normalize.argtypes = [C.c_uint64, f64p, f64p, C.c_int, C.c_uint64, f64p, f64p, C.c_double]
+        L.oracle_tube_spectrum.argtypes = [C.POINTER(AnodeTables), C.c_double, C.c_double, f64p, C.c_uint64, i32p, f64p, C.c_int32, C.c_int32, f64p]
     return _lib
+
+
+class AnodeTables(C.Structure):
+    _fields_ = [("u", f64p), ("nu", C.c_int32), ("depth_f", f64p), ("depth_m", f64p), ("nx", C.c_int32), ("density_f", f64p),
+                ("density_m", f64p), ("tw_voltage", f64p), ("tw_constant", f64p), ("ntw", C.c_int32), ("k_energy", f64p),
+                ("k_fraction", f64p), ("nk", C.c_int32)]
+
+
+def tube_spectrum(data, kvp, takeoff, energies, filters=(), normalize=True):
+    """The reference's tube spectrum by its own nested-loop integral, on the
+    tables in 'data' (a mapping of the fixture's arrays). filters: (Z, mm)
+    applied in order."""
+    keep = {k: np.ascontiguousarray(data[k], dtype=np.float64) for k in
+            ("u", "depth_f", "depth_m", "density_f", "density_m", "tw_voltage", "tw_constant", "k_energy", "k_fraction")}
+    t = AnodeTables(_p(keep["u"], C.c_double), len(keep["u"]), _p(keep["depth_f"], C.c_double), _p(keep["depth_m"], C.c_double),
+                    len(keep["depth_f"]), _p(keep["density_f"], C.c_double), _p(keep["density_m"], C.c_double),
+                    _p(keep["tw_voltage"], C.c_double), _p(keep["tw_constant"], C.c_double), len(keep["tw_voltage"]),
+                    _p(keep["k_energy"], C.c_double), _p(keep["k_fraction"], C.c_double), len(keep["k_energy"]))
+    e = np.ascontiguousarray(energies, dtype=np.float64)
+    z = np.array([f[0] for f in filters], dtype=np.int32)
+    mm = np.array([f[1] for f in filters], dtype=np.float64)
+    out = np.zeros(len(e))
+    lib().oracle_tube_spectrum(C.byref(t), kvp, takeoff, _p(e, C.c_double), len(e), _p(z, C.c_int32), _p(mm, C.c_double), len(filters),
+                               int(normalize), _p(out, C.c_double))
+    return out
 
 
 def ref():

@@ -27,6 +27,9 @@
 //   Woodcock tracking                      uncollided transmission exp(-mu L),
 //                                          majorant independence, energy balance
 //   normalisation                          closed form
+//   tube spectrum (Bethe-Heitler anode)     literal nested-loop integral on
+//                                          the tables of a fixture extracted
+//                                          from the reference; TG-195 RQR-8/9
 #include "dxmc/material.hpp"
 #include "dxmc_hip.h"
 
@@ -1293,6 +1296,145 @@ void oracle_normalize(std::uint64_t n, double* dose, double* variance, int mode,
         const double f = calibration / mass;
         dose[i] = density[i] > 0 ? dose[i] * f : 0;
         variance[i] = density[i] > 0 ? variance[i] * f * f : 0;
+    }
+}
+}
+
+// ---------------------------------------------------------------- tube spectrum
+// Restatement of the reference's tube spectrum as the reference evaluates it:
+// betheHeitlerCrossSection.hpp:174-391 (nested x, u loops per photon energy,
+// electron density re-interpolated at every node), tube.hpp:189-258 (K lines
+// at the first bin >= the line within 2 keV, filtration, normalisation).
+// The electron-penetration tables come in as arrays (from a fixture), so the
+// product's own copy of them is checked too.
+namespace {
+struct AnodeTables {
+    const double* u;
+    std::int32_t nu;
+    const double* depthF;
+    const double* depthM;
+    std::int32_t nx;
+    const double* densityF; // [nu][nx]
+    const double* densityM;
+    const double* twVoltage;
+    const double* twConstant;
+    std::int32_t ntw;
+    const double* kEnergy;
+    const double* kFraction;
+    std::int32_t nk;
+};
+
+// betheHeitlerCrossSection.hpp:182-206 and the segment rule of 240-252
+std::int64_t refSegment(const double* g, std::int64_t n, double v)
+{
+    std::int64_t i = std::lower_bound(g, g + n, v) - g;
+    const std::int64_t rest = n - i;
+    if (rest < 2)
+        i = n + (-3 + rest);
+    return i;
+}
+
+double refTwLaw(const AnodeTables& t, double x, double T0)
+{
+    const std::int64_t i = refSegment(t.twVoltage, t.ntw, T0);
+    const double C = t.twConstant[i] + ((t.twConstant[i + 1] - t.twConstant[i]) / (t.twVoltage[i + 1] - t.twVoltage[i])) * (T0 - t.twVoltage[i]);
+    const double twl = (T0 * T0 - C * x) / (T0 * T0);
+    return twl < 0 ? 0 : twl;
+}
+double refRange(double T0) { return 0.0119 * std::pow(T0, 1.513); } // :175-179
+double refFracF(const AnodeTables& t, double x, double T0) { return std::pow(refTwLaw(t, x, T0), 1.753); } // :208-213
+double refFracM(const AnodeTables& t, double x, double T0) // :214-224
+{
+    const double K = 18.0, Bd = 0.584, B0 = 0.5;
+    const double ex = 1 - std::exp(-K * x / refRange(T0));
+    const double F = Bd * ex;
+    const double B = B0 + (Bd - B0) * ex;
+    return refFracF(t, x, T0) * B * (F + 1) / (1 - B * F);
+}
+// :226-311 (bilinear, clamped, row u / column depth)
+double refDensity(const AnodeTables& t, const double* depth, const double* table, double uval, double xval)
+{
+    const double x = std::clamp(xval, depth[0], depth[t.nx - 1]);
+    const double u = std::clamp(uval, t.u[0], t.u[t.nu - 1]);
+    const std::int64_t ix = refSegment(depth, t.nx, x), iu = refSegment(t.u, t.nu, u);
+    const double q11 = table[iu * t.nx + ix], q21 = table[iu * t.nx + ix + 1];
+    const double q22 = table[(iu + 1) * t.nx + ix + 1], q12 = table[(iu + 1) * t.nx + ix];
+    const double x1 = depth[ix], x2 = depth[ix + 1], y1 = t.u[iu], y2 = t.u[iu + 1];
+    const double xf1 = ((x2 - x) / (x2 - x1));
+    const double xf2 = ((x - x1) / (x2 - x1));
+    const double r1 = xf1 * q11 + xf2 * q21;
+    const double r2 = xf1 * q12 + xf2 * q22;
+    return ((y2 - u) / (y2 - y1)) * r1 + ((u - y1) / (y2 - y1)) * r2;
+}
+double refElectronDensity(const AnodeTables& t, double u, double x, double T0) // :313-318
+{
+    const double f = refRange(100.0) / refRange(T0);
+    return refFracF(t, x, T0) * refDensity(t, t.depthF, t.densityF, u, x * f) + refFracM(t, x, T0) * refDensity(t, t.depthM, t.densityM, u, x * f);
+}
+double refBetheHeitler(double hv, double Ti) // :349-366
+{
+    const double m = 510.9989461;
+    const double phiBar = (74 * 74) * 2.81794092E-15 * 2.81794092E-15 * 7.29735308E-03;
+    const double k = (phiBar * 2) / 3;
+    const double Ei = m + Ti;
+    const double Ef = Ei - hv;
+    const double pic_sqr = Ei * Ei - m * m;
+    const double pic = std::sqrt(pic_sqr);
+    const double pfc_sqr = Ef * Ef - m * m;
+    if (pfc_sqr <= 0)
+        return 0;
+    const double pfc = std::sqrt(pfc_sqr);
+    const double L = 2 * std::log((Ei * Ef + pic * pfc - m * m) / (m * hv));
+    const double cc = pic / pfc;
+    return k * (4 * Ei * Ef * L - 7 * pic * pfc) / (hv * pic * pic) * cc;
+}
+double refBremsstrahlung(const AnodeTables& t, double T0, double hv, double takeoff) // :368-392
+{
+    const double att = dxmc::Material::getTotalAttenuation(74, hv);
+    double x = 0, I = 0;
+    if (hv <= 0)
+        return 0;
+    while (x <= 14.0) {
+        double step = 0;
+        double u = 0.005;
+        while (u <= 1.0) {
+            step = step + refBetheHeitler(hv, T0 * u) * refElectronDensity(t, u, x, T0) * 0.005;
+            u = u + 0.005;
+        }
+        I = I + step * std::exp(-att * x * 0.001 / std::sin(takeoff)) * 0.1;
+        x = x + 0.1;
+    }
+    return I;
+}
+}
+
+extern "C" {
+// tube.hpp:189-208 with the filters applied in order; out[n]
+void oracle_tube_spectrum(const AnodeTables* t, double T0, double takeoff, const double* energies, std::uint64_t n, const std::int32_t* filterZ,
+    const double* filterMm, std::int32_t nFilters, std::int32_t normalize, double* out)
+{
+    for (std::uint64_t i = 0; i < n; ++i)
+        out[i] = refBremsstrahlung(*t, T0, energies[i], takeoff);
+    // tube.hpp:216-236, betheHeitlerCrossSection.hpp:394-407
+    for (std::int32_t k = 0; k < t->nk; ++k) {
+        const double e = t->kEnergy[k];
+        const double intensity = (1 + 0.33) * t->kFraction[k] * 4.4 * 0.94 * refBremsstrahlung(*t, T0, e, takeoff);
+        const double* it = std::lower_bound(energies, energies + n, e);
+        if (it != energies + n && std::abs(e - *it) <= 2.0)
+            out[it - energies] = out[it - energies] + intensity;
+    }
+    for (std::int32_t f = 0; f < nFilters; ++f) { // :237-244
+        const dxmc::Material mat(filterZ[f]);
+        const double cm = filterMm[f] * 0.1;
+        for (std::uint64_t i = 0; i < n; ++i)
+            out[i] = out[i] * std::exp(-mat.getTotalAttenuation(energies[i]) * mat.standardDensity() * cm);
+    }
+    if (normalize) { // :245-249, summed in order
+        double sum = 0;
+        for (std::uint64_t i = 0; i < n; ++i)
+            sum = sum + out[i];
+        for (std::uint64_t i = 0; i < n; ++i)
+            out[i] = out[i] / sum;
     }
 }
 }
