@@ -121,7 +121,17 @@ struct Exposure {
 // (x = FP32 density bits, y = material | measurement << 8) in 'palette' and
 // a 1- or 2-byte palette index per voxel in 'voxels' (voxelBytes 1 or 2);
 // worlds with more than 65536 distinct records keep the 8-byte record itself
-// per voxel (voxelBytes 8, palette unused).
+// per voxel (voxelBytes 8, palette unused). Such a world may also carry a
+// 1- or 2-byte code per voxel (codeBytes 1 or 2, else 0) that names the
+// voxel's material and an interval holding its density ('bounds'): the
+// Woodcock decision is settled from the interval at most steps, and the
+// 8-byte record is read only when the draw falls inside the interval's
+// acceptance band.
+struct DensityBound {
+    float lo, hi; // lo <= FP32 density <= hi for every voxel with this code
+    uint32_t material, pad;
+};
+
 template <typename T>
 struct World {
     uint32_t nx, ny, nz, voxelBytes;
@@ -132,6 +142,9 @@ struct World {
     const uint8_t* measurement; // may be null
     const void* voxels;
     const uint2* palette;
+    const void* codes;
+    const DensityBound* bounds;
+    uint32_t codeBytes, pad;
 };
 
 struct Tally {

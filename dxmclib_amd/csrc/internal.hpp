@@ -184,6 +184,12 @@ struct dxmc_world {
     void* palette = nullptr; // uint2[nPalette] FP32 records, null for 8-byte voxels
     uint32_t voxelBytes = 8;
     uint32_t nPalette = 0;
+    // 8-byte-record worlds: bounded-density code per voxel (u8/u16) and the
+    // interval of each code (DensityBound[nCodes]); codeBytes 0: none
+    void* codes = nullptr;
+    void* bounds = nullptr;
+    uint32_t codeBytes = 0;
+    uint32_t nCodes = 0;
     uint64_t bytes = 0;
 };
 
@@ -280,6 +286,9 @@ inline dxmc_dev::World<T> deviceWorld(const dxmc_world* w)
     d.measurement = static_cast<const uint8_t*>(w->measurement);
     d.voxels = w->voxels;
     d.palette = static_cast<const uint2*>(w->palette);
+    d.codes = w->codes;
+    d.bounds = static_cast<const dxmc_dev::DensityBound*>(w->bounds);
+    d.codeBytes = w->codeBytes;
     d.voxelBytes = w->voxelBytes;
     return d;
 }

@@ -320,12 +320,34 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(W, 8))) voi
                 p.pos[i] += p.dir[i] * step;
             const bool in = insideFast(w, p.pos);
             idx = voxelIndexFast(w, p.pos, inv); // clamped: a valid voxel also outside
-            readVoxel(w, idx, mat, density, meas);
-            float att[3];
-            attenuationAt(t, mat, lp, att);
-            const float prob = ((att[0] + att[1]) + att[2]) * density * majInv;
             Rng r2 = rng;
             const float u2 = r2.uniform<float>();
+            float att[3];
+            float prob;
+            if (w.codeBytes != 0) {
+                // the voxel's code gives its material and lo <= density <= hi;
+                // the acceptance probability is monotone in the density, so
+                // u2 < p(lo) accepts and u2 >= p(hi) rejects exactly as the
+                // density itself would; in between, the record decides
+                const uint32_t code = w.codeBytes == 1 ? static_cast<const uint8_t*>(w.codes)[idx] : static_cast<const uint16_t*>(w.codes)[idx];
+                const DensityBound b = w.bounds[code];
+                mat = b.material;
+                meas = 0;
+                attenuationAt(t, mat, lp, att);
+                const float sum = (att[0] + att[1]) + att[2];
+                const float lo = sum * b.lo * majInv;
+                const float hi = sum * b.hi * majInv;
+                density = b.lo;
+                prob = lo;
+                if (u2 >= lo && u2 < hi) {
+                    density = __uint_as_float(static_cast<const uint2*>(w.voxels)[idx].x);
+                    prob = sum * density * majInv;
+                }
+            } else {
+                readVoxel(w, idx, mat, density, meas);
+                attenuationAt(t, mat, lp, att);
+                prob = ((att[0] + att[1]) + att[2]) * density * majInv;
+            }
             const bool drawn = in & (meas == 0);
             rng.s = drawn ? r2.s : rng.s;
             const bool real = (meas != 0) | (u2 < prob);
@@ -429,7 +451,10 @@ void launchFp32(dxmc_ctx* c, const dxmc_run_cfg& cfg, const World<float>& w, con
     uint64_t grid = static_cast<uint64_t>(c->numCU) * perCU;
     const uint64_t lanesNeeded = (he - hb + 63) / 64 * 64;
     grid = std::max<uint64_t>(1, std::min<uint64_t>(grid, (lanesNeeded + block - 1) / block));
-    hipLaunchKernelGGL(kernelFor(cfg.low_energy_correction, waves, stage), dim3(static_cast<uint32_t>(grid)), dim3(block), lds, c->stream, w, t,
+    World<float> world = w;
+    if (cfg.flags & DXMC_RUN_FULL_RECORDS)
+        world.codeBytes = 0;
+    hipLaunchKernelGGL(kernelFor(cfg.low_energy_correction, waves, stage), dim3(static_cast<uint32_t>(grid)), dim3(block), lds, c->stream, world, t,
         exps, expStart, nExp, hb, he, cfg.seed, cfg.first_history, tally, ctr, nx, regen, interact, pool);
 }
 }

@@ -14,12 +14,16 @@
 //   paletteIndex<I> 1- or 2-byte palette index of every voxel
 //   packVoxels      8-byte records when there are more than 65536
 //                   distinct records
+//   boundCodes<I>   for those worlds, a 1- or 2-byte code per voxel naming
+//                   its material and a density interval (Woodcock decisions
+//                   settled from the interval at most steps)
 // The host reads back a few hundred bytes (maxima, flags, distinct count),
 // so the cost of an upload is the copy and a few HBM passes.
 #include "dxmc_hip.h"
 #include "internal.hpp"
 
 #include <algorithm>
+#include <cmath>
 #include <string>
 #include <type_traits>
 #include <vector>
@@ -65,21 +69,24 @@ __device__ inline unsigned long long recordKey(const uint8_t* mat, const double*
 __device__ inline uint32_t hashSlot(unsigned long long k) { return static_cast<uint32_t>((k * 0x9E3779B97F4A7C15ull) >> (64 - HASH_BITS)); }
 
 // flags[0]: largest material index; flags[1]: any measurement flag set.
-// maxKey[m]: order key of material m's largest density (0: no voxel).
+// maxKey[m]: order key of material m's largest density, minKeyInv[m]: the
+// complement of the order key of its smallest (0 for both: no voxel).
 template <typename D>
 __global__ __launch_bounds__(SCAN_BLOCK) void scanWorld(const uint8_t* __restrict__ mat, const D* __restrict__ dens, double* __restrict__ dens64,
-    const uint8_t* __restrict__ meas, uint64_t n, unsigned long long* __restrict__ maxKey, unsigned int* __restrict__ flags)
+    const uint8_t* __restrict__ meas, uint64_t n, unsigned long long* __restrict__ maxKey, unsigned long long* __restrict__ minKeyInv,
+    unsigned int* __restrict__ flags)
 {
-    __shared__ unsigned long long key[SCAN_BLOCK];
+    __shared__ unsigned long long key[SCAN_BLOCK], keyInv[SCAN_BLOCK];
     __shared__ unsigned int blockMat, blockMeas;
     key[threadIdx.x] = 0;
+    keyInv[threadIdx.x] = 0;
     if (threadIdx.x == 0) {
         blockMat = 0;
         blockMeas = 0;
     }
     __syncthreads();
     uint32_t lastMat = SCAN_BLOCK, largestMat = 0, anyMeas = 0;
-    unsigned long long lastKey = 0;
+    unsigned long long lastKey = 0, lastKeyInv = 0;
     const uint64_t stride = static_cast<uint64_t>(gridDim.x) * SCAN_BLOCK;
     for (uint64_t i = static_cast<uint64_t>(blockIdx.x) * SCAN_BLOCK + threadIdx.x; i < n; i += stride) {
         const double v = static_cast<double>(dens[i]);
@@ -91,12 +98,23 @@ __global__ __launch_bounds__(SCAN_BLOCK) void scanWorld(const uint8_t* __restric
             anyMeas |= meas[i];
         if (v == v) {
             const unsigned long long k = orderKey(v);
-            // skip the LDS atomic when this thread already posted a larger
-            // value for the same material
-            if (m != lastMat || k > lastKey) {
+            // skip the LDS atomics when this thread already posted a larger
+            // and a smaller value for the same material
+            if (m != lastMat) {
                 atomicMax(&key[m], k);
+                atomicMax(&keyInv[m], ~k);
                 lastMat = m;
                 lastKey = k;
+                lastKeyInv = ~k;
+            } else {
+                if (k > lastKey) {
+                    atomicMax(&key[m], k);
+                    lastKey = k;
+                }
+                if (~k > lastKeyInv) {
+                    atomicMax(&keyInv[m], ~k);
+                    lastKeyInv = ~k;
+                }
             }
         }
     }
@@ -104,8 +122,10 @@ __global__ __launch_bounds__(SCAN_BLOCK) void scanWorld(const uint8_t* __restric
     if (anyMeas)
         atomicOr(&blockMeas, 1u);
     __syncthreads();
-    if (key[threadIdx.x])
+    if (key[threadIdx.x]) {
         atomicMax(&maxKey[threadIdx.x], key[threadIdx.x]);
+        atomicMax(&minKeyInv[threadIdx.x], keyInv[threadIdx.x]);
+    }
     if (threadIdx.x == 0) {
         atomicMax(&flags[0], blockMat);
         if (blockMeas)
@@ -219,12 +239,122 @@ __global__ __launch_bounds__(SCAN_BLOCK) void packVoxels(const uint8_t* __restri
     }
 }
 
+// Code of every voxel of an 8-byte-record world: the bin of its FP32
+// density among the B bins of its material, base[m] + k, checked against
+// the interval it names (bounds[code].lo <= density <= bounds[code].hi); a
+// voxel for which that fails sets *failed and the codes are not used.
+template <typename I>
+__global__ __launch_bounds__(SCAN_BLOCK) void boundCodes(const uint2* __restrict__ records, uint64_t n, const dxmc_dev::DensityBound* __restrict__ bounds,
+    const uint32_t* __restrict__ base, const double* __restrict__ lower, const double* __restrict__ scale, uint32_t bins, I* __restrict__ codes,
+    unsigned int* failed)
+{
+    const uint64_t stride = static_cast<uint64_t>(gridDim.x) * SCAN_BLOCK;
+    for (uint64_t i = static_cast<uint64_t>(blockIdx.x) * SCAN_BLOCK + threadIdx.x; i < n; i += stride) {
+        const uint2 r = records[i];
+        const float d = __uint_as_float(r.x);
+        const uint32_t m = r.y & 0xffu;
+        const double x = (static_cast<double>(d) - lower[m]) * scale[m];
+        int k = x > 0 ? static_cast<int>(min(x, static_cast<double>(bins - 1))) : 0;
+        const dxmc_dev::DensityBound* b = bounds + base[m];
+        for (uint32_t guard = 0; guard < bins && k > 0 && d < b[k].lo; ++guard)
+            --k;
+        for (uint32_t guard = 0; guard < bins && k + 1 < static_cast<int>(bins) && d > b[k].hi; ++guard)
+            ++k;
+        if (!(b[k].lo <= d && d <= b[k].hi && b[k].material == m))
+            atomicOr(failed, 1u);
+        codes[i] = static_cast<I>(base[m] + static_cast<uint32_t>(k));
+    }
+}
+
 // Scan results read back by the host.
 struct ScanResult {
     unsigned long long maxKey[SCAN_BLOCK];
+    unsigned long long minKeyInv[SCAN_BLOCK];
     unsigned int flags[2];
     unsigned int palette[2]; // distinct records, overflow
+    unsigned int codesFailed, pad;
 };
+
+float floatBelow(double v)
+{
+    float f = static_cast<float>(v);
+    if (static_cast<double>(f) > v)
+        f = std::nextafter(f, -HUGE_VALF);
+    return f;
+}
+float floatAbove(double v)
+{
+    float f = static_cast<float>(v);
+    if (static_cast<double>(f) < v)
+        f = std::nextafter(f, HUGE_VALF);
+    return f;
+}
+
+// Bounded-density codes of an 8-byte-record world without measurement
+// flags: the materials present share the 256 (1-byte codes, up to 64
+// materials) or 65536 (2-byte codes) codes equally; each material's
+// [min, max] density range is cut into that many equal bins, each widened
+// outwards to FP32 values. Leaves codeBytes 0 when a voxel's density does
+// not fall in the bin computed for it.
+int buildBoundCodes(dxmc_ctx* c, dxmc_world* w, ScanResult* dev, const ScanResult& host)
+{
+    std::vector<uint32_t> present;
+    for (uint32_t m = 0; m < w->nMaterials; ++m)
+        if (host.maxKey[m])
+            present.push_back(m);
+    if (present.empty() || w->hasMeasurement)
+        return DXMC_OK;
+    const uint32_t codeBytes = present.size() <= 64 ? 1u : 2u;
+    const uint32_t nCodes = codeBytes == 1 ? 256u : 65536u;
+    const uint32_t bins = nCodes / static_cast<uint32_t>(present.size());
+    std::vector<dxmc_dev::DensityBound> table(nCodes, dxmc_dev::DensityBound { 1.0f, 0.0f, 0xffffffffu, 0u });
+    std::vector<uint32_t> base(SCAN_BLOCK, 0);
+    std::vector<double> lower(SCAN_BLOCK, 0.0), scale(SCAN_BLOCK, 0.0);
+    for (uint32_t j = 0; j < present.size(); ++j) {
+        const uint32_t m = present[j];
+        // FP32 records hold float(density): their range is the rounded range
+        const float a = static_cast<float>(fromOrderKey(~host.minKeyInv[m]));
+        const float b = static_cast<float>(fromOrderKey(host.maxKey[m]));
+        base[m] = j * bins;
+        lower[m] = a;
+        scale[m] = b > a ? bins / (static_cast<double>(b) - a) : 0.0;
+        for (uint32_t k = 0; k < bins; ++k) {
+            auto& e = table[base[m] + k];
+            const double e0 = a + (static_cast<double>(b) - a) * k / bins;
+            const double e1 = a + (static_cast<double>(b) - a) * (k + 1) / bins;
+            e.lo = k == 0 ? a : floatBelow(e0);
+            e.hi = k + 1 == bins ? b : floatAbove(e1);
+            e.material = m;
+        }
+    }
+    DeviceBuffer dBase(c), dLower(c), dScale(c);
+    int st = DXMC_OK;
+    if ((st = c->alloc(&w->bounds, nCodes * sizeof(dxmc_dev::DensityBound))) || (st = c->alloc(&w->codes, w->n * codeBytes))
+        || (st = c->copy(w->bounds, table.data(), nCodes * sizeof(dxmc_dev::DensityBound))) || (st = dBase.upload(base.data(), base.size() * 4))
+        || (st = dLower.upload(lower.data(), lower.size() * 8)) || (st = dScale.upload(scale.data(), scale.size() * 8)))
+        return st;
+    const auto* rec = static_cast<const uint2*>(w->voxels);
+    const auto* bnd = static_cast<const dxmc_dev::DensityBound*>(w->bounds);
+    const dim3 grid(gridFor(w->n, SCAN_BLOCK, SCAN_GRID)), block(SCAN_BLOCK);
+    if (codeBytes == 1)
+        hipLaunchKernelGGL(boundCodes<uint8_t>, grid, block, 0, c->stream, rec, w->n, bnd, static_cast<const uint32_t*>(dBase.p),
+            static_cast<const double*>(dLower.p), static_cast<const double*>(dScale.p), bins, static_cast<uint8_t*>(w->codes), &dev->codesFailed);
+    else
+        hipLaunchKernelGGL(boundCodes<uint16_t>, grid, block, 0, c->stream, rec, w->n, bnd, static_cast<const uint32_t*>(dBase.p),
+            static_cast<const double*>(dLower.p), static_cast<const double*>(dScale.p), bins, static_cast<uint16_t*>(w->codes), &dev->codesFailed);
+    unsigned int failed = 1;
+    if ((st = c->check(hipGetLastError())) || (st = c->download(&failed, &dev->codesFailed, sizeof(failed))))
+        return st;
+    if (failed) {
+        c->release(w->codes);
+        c->release(w->bounds);
+        w->codes = w->bounds = nullptr;
+        return DXMC_OK;
+    }
+    w->codeBytes = codeBytes;
+    w->nCodes = nCodes;
+    return DXMC_OK;
+}
 
 int fillVoxelRecords(dxmc_ctx* c, dxmc_world* w, ScanResult* dev, ScanResult& host)
 {
@@ -262,6 +392,8 @@ int fillVoxelRecords(dxmc_ctx* c, dxmc_world* w, ScanResult* dev, ScanResult& ho
             return st;
         hipLaunchKernelGGL(packVoxels, dim3(gridFor(n, SCAN_BLOCK, SCAN_GRID)), dim3(SCAN_BLOCK), 0, c->stream, mat, dens, meas,
             static_cast<uint2*>(w->voxels), n);
+        if ((st = c->check(hipGetLastError())) || (st = buildBoundCodes(c, w, dev, host)))
+            return st;
     }
     if ((st = c->check(hipGetLastError())))
         return st;
@@ -316,12 +448,12 @@ int dxmc_hip_world_upload(dxmc_ctx* c, const dxmc_world_desc* d, dxmc_world** ou
     if (d->density_f64) {
         if ((st = c->copy(w->density, d->density_f64, n * sizeof(double))))
             return fail(st);
-        hipLaunchKernelGGL(scanWorld<double>, grid, block, 0, c->stream, mat, dens64, dens64, meas, n, res->maxKey, res->flags);
+        hipLaunchKernelGGL(scanWorld<double>, grid, block, 0, c->stream, mat, dens64, dens64, meas, n, res->maxKey, res->minKeyInv, res->flags);
     } else {
         if ((st = dens32.upload(d->density_f32, n * sizeof(float))))
             return fail(st);
         hipLaunchKernelGGL(scanWorld<float>, grid, block, 0, c->stream, mat, static_cast<const float*>(dens32.p), dens64, meas, n, res->maxKey,
-            res->flags);
+            res->minKeyInv, res->flags);
     }
     ScanResult host {};
     if ((st = c->check(hipGetLastError())) || (st = c->download(&host, res, sizeof(ScanResult))))
@@ -339,7 +471,8 @@ int dxmc_hip_world_upload(dxmc_ctx* c, const dxmc_world_desc* d, dxmc_world** ou
         w->maxDensity[m] = std::max(0.0, fromOrderKey(host.maxKey[m]));
     if ((st = fillVoxelRecords(c, w, res, host)))
         return fail(st);
-    w->bytes = n * (1 + sizeof(double) + w->voxelBytes + (w->hasMeasurement ? 1 : 0)) + w->nPalette * sizeof(uint2);
+    w->bytes = n * (1 + sizeof(double) + w->voxelBytes + (w->hasMeasurement ? 1 : 0) + w->codeBytes) + w->nPalette * sizeof(uint2)
+        + w->nCodes * sizeof(dxmc_dev::DensityBound);
     *out = w;
     return DXMC_OK;
 }
@@ -358,6 +491,8 @@ int dxmc_hip_world_destroy(dxmc_ctx* c, dxmc_world* w)
     c->release(w->measurement);
     c->release(w->voxels);
     c->release(w->palette);
+    c->release(w->codes);
+    c->release(w->bounds);
     hipStreamSynchronize(c->stream);
     delete w;
     return DXMC_OK;

@@ -259,7 +259,7 @@ def test_fp32_voxel_records(device, records):
     else:
         dens = (base * rng.uniform(0.98, 1.02, shape)).astype(np.float32)
         assert np.unique(np.stack([dens.ravel().view(np.uint32), mat.ravel().astype(np.uint32)]), axis=1).shape[1] > 65536
-        distinct, vb = 0, 8
+        distinct, vb = 0, 8 + 1  # the record and a 1-byte bounded-density code
     # pencil through voxel centres (a beam along voxel faces lets the FP32
     # and FP64 paths pick different voxels for photons within an FP32 ulp of
     # the face, so their histories decorrelate)
@@ -270,7 +270,7 @@ def test_fp32_voxel_records(device, records):
     w = device.upload_world(mat, dens, wl.spacing, ext, 2)
     try:
         n = mat.size
-        assert device.world_bytes(w) == n * (1 + 8 + vb) + 8 * distinct
+        assert device.world_bytes(w) == n * (1 + 8 + vb) + 8 * distinct + (16 * 256 if records == "direct" else 0)
     finally:
         device.destroy_world(w)
     d64, _, v64, s64, *_ = _run_device(device, wl, 1, capi.PRECISION_FP64)
@@ -283,6 +283,53 @@ def test_fp32_voxel_records(device, records):
     assert frac >= 0.997
     assert abs(s32.steps - s64.steps) <= 1e-3 * s64.steps
     assert abs(s32.events - s64.events) <= 1e-3 * s64.events
+
+
+@pytest.mark.parametrize("n_materials", [3, 70])
+def test_bounded_density_codes_exact(device, n_materials):
+    """A world with 8-byte voxel records also gets a 1-byte (up to 64
+    materials) or 2-byte code per voxel: its material and an interval that
+    holds its density. The FP32 kernel settles each Woodcock decision from
+    the interval where it can and reads the record otherwise; the transport
+    is identical to reading every record (DXMC_RUN_FULL_RECORDS)."""
+    print(f"bounded density codes, {n_materials} materials", flush=True)
+    rng = np.random.default_rng(8)
+    shape = (48, 48, 48)
+    names = (NAMES * 20)[:n_materials]
+    dens_of = np.array((MAXD * 20)[:n_materials], dtype=np.float32)
+    dens_of[dens_of < 0.01] = 0.5  # no near-vacuum: every material interacts
+    mat = rng.integers(0, n_materials, shape).astype(np.uint8)
+    dens = (dens_of[mat] * rng.uniform(0.9, 1.1, shape)).astype(np.float32)
+    spacing = (1.0, 1.0, 1.0)
+    ext = backend.world_extent(shape[::-1], spacing)
+    w = device.upload_world(mat, dens, spacing, ext, n_materials)
+    code_bytes = 1 if n_materials <= 64 else 2
+    n = mat.size
+    assert device.world_bytes(w) == n * (1 + 8 + 8 + code_bytes) + 16 * (256 if code_bytes == 1 else 65536)
+    maxd = device.world_max_density(w, n_materials)
+    t = backend.HostTables(names, maxd, 120.0, 1.0)
+    spec = backend.source_spec(capi.SOURCE_DX, tube_voltage=120.0, al_mm=2.5, sdd=1000.0, field_size=(48.0, 48.0),
+                               histories_per_exposure=20_000, total_exposures=10)
+    exps = t.exposures(spec)
+    tt = device.upload_tables(t.desc())
+    out = []
+    try:
+        for flags in (capi.RUN_COUNTERS, capi.RUN_COUNTERS | capi.RUN_FULL_RECORDS):
+            cfg = capi.RunCfg()
+            cfg.flags = flags
+            tally = device.tally(n)
+            stats = device.run(w, tt, exps, tally, capi.PRECISION_FP32, 1, seed=3, tuning=cfg)
+            d, ev, v = device.download(tally, n)
+            device.destroy_tally(tally)
+            out.append((stats.steps, stats.events, d, ev))
+    finally:
+        device.destroy_tables(tt)
+        device.destroy_world(w)
+    print(f"  steps {out[0][0]} vs {out[1][0]}, events {out[0][1]} vs {out[1][1]}")
+    assert out[0][0] == out[1][0] and out[0][1] == out[1][1] and out[0][0] > 0
+    assert np.array_equal(out[0][3], out[1][3])
+    # FP64 atomics add in run-dependent order: rounding only
+    assert np.allclose(out[0][2], out[1][2], rtol=1e-10, atol=0)
 
 
 @pytest.mark.parametrize("dtype", [np.float32, np.float64])
@@ -304,17 +351,19 @@ def test_world_scan_on_device(device, dtype):
     def upload(meas=None, n_materials=5):
         return device.upload_world(mat, dens, spacing, ext, n_materials, measurement=meas)
 
+    # 8-byte records plus 1-byte bounded-density codes (256 intervals)
     w = upload()
     try:
         assert np.array_equal(device.world_max_density(w, 5), want)
-        assert device.world_bytes(w) == n * (1 + 8 + 8)
+        assert device.world_bytes(w) == n * (1 + 8 + 8 + 1) + 16 * 256
     finally:
         device.destroy_world(w)
-    # an all-zero measurement map is dropped; one flag keeps it
+    # an all-zero measurement map is dropped; one flag keeps it (and then
+    # no codes: forced interactions read the record)
     meas = np.zeros(shape, np.uint8)
     w = upload(meas)
     try:
-        assert device.world_bytes(w) == n * (1 + 8 + 8)
+        assert device.world_bytes(w) == n * (1 + 8 + 8 + 1) + 16 * 256
     finally:
         device.destroy_world(w)
     meas[3, 4, 5] = 1
