@@ -263,6 +263,33 @@ class Device:
                                           cancel, progress, C.byref(stats)))
         return stats
 
+    def run_monitored(self, world, tables, exposures, tally, precision, low_energy_correction=1, seed=42, tuning=None, preview=None,
+                      preview_axis=2, preview_interval=0.0, cancel=None, progress=None):
+        """run() with a live preview: preview(image) receives the projection
+        of the tally along preview_axis (a 2-D float32 array, rows along the
+        second remaining axis) between launches."""
+        cfg = tuning if tuning is not None else capi.RunCfg()
+        cfg.precision = precision
+        cfg.low_energy_correction = low_energy_correction
+        cfg.seed = seed
+
+        def on_preview(_user, image, width, height):
+            preview(np.ctypeslib.as_array(image, shape=(height, width)).copy())
+
+        fn = capi.PreviewFn(on_preview) if preview is not None else capi.PreviewFn()
+        mon = capi.RunMonitor(cancel, progress, fn, None, preview_axis, preview_interval)
+        stats = capi.RunStats()
+        self._check(self.lib.dxmc_hip_run_monitored(self.ctx, world, tables, exposures, len(exposures), C.byref(cfg), tally, C.byref(mon),
+                                                    C.byref(stats)))
+        return stats
+
+    def tally_mip(self, tally, world, dims, axis):
+        """Maximum of the tally's dose along axis (0 x, 1 y, 2 z); dims (nx, ny, nz)."""
+        other = [d for i, d in enumerate(dims) if i != axis]
+        out = np.zeros(other[0] * other[1], dtype=np.float32)
+        self._check(self.lib.dxmc_hip_tally_mip(self.ctx, tally, world, axis, out.ctypes.data_as(C.POINTER(C.c_float))))
+        return out.reshape(other[1], other[0])
+
     def autotune(self, world, tables, exposures, precision, low_energy_correction=1, trial_histories=1 << 23, base=None):
         """dxmc_hip_autotune: tuned run configuration and its report."""
         cfg = base if base is not None else capi.RunCfg()

@@ -162,6 +162,20 @@ class RunStats(C.Structure):
     ]
 
 
+PreviewFn = C.CFUNCTYPE(None, C.c_void_p, C.POINTER(C.c_float), C.c_uint64, C.c_uint64)
+
+
+class RunMonitor(C.Structure):
+    _fields_ = [
+        ("cancel", C.POINTER(C.c_int32)),
+        ("exposures_done", C.POINTER(C.c_uint64)),
+        ("preview", PreviewFn),
+        ("preview_user", C.c_void_p),
+        ("preview_axis", C.c_int32),
+        ("preview_interval_seconds", C.c_float),
+    ]
+
+
 class SourceSpec(C.Structure):
     _fields_ = [
         ("type", C.c_int32),
@@ -289,6 +303,9 @@ HIP_SYMBOLS = {
     "dxmc_hip_tally_normalize": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int, C.c_uint64, C.c_double]),
     "dxmc_hip_run": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.POINTER(Exposure), C.c_uint64, C.POINTER(RunCfg), C.c_void_p,
                                C.POINTER(C.c_int32), C.POINTER(C.c_uint64), C.POINTER(RunStats)]),
+    "dxmc_hip_run_monitored": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.POINTER(Exposure), C.c_uint64, C.POINTER(RunCfg), C.c_void_p,
+                                         C.POINTER(RunMonitor), C.POINTER(RunStats)]),
+    "dxmc_hip_tally_mip": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int32, C.POINTER(C.c_float)]),
     "dxmc_hip_autotune": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.POINTER(Exposure), C.c_uint64, C.POINTER(RunCfg), C.c_uint64,
                                     C.POINTER(RunCfg), C.POINTER(TuneReport)]),
     "dxmc_hip_rng_streams": (C.c_int, [C.c_void_p, C.c_uint64, C.c_uint64, C.c_uint64, C.c_uint32, u32p]),
@@ -306,6 +323,8 @@ HOST_SYMBOLS = {
     "dxmc_host_last_error": (C.c_char_p, []),
     "dxmc_host_source_exposures": (C.c_int, [C.POINTER(SourceSpec), f64p, C.c_void_p, C.POINTER(Exposure), C.c_uint64, C.POINTER(C.c_uint64)]),
     "dxmc_host_source_spectrum": (C.c_int, [C.POINTER(SourceSpec), f64p, f64p, C.c_uint64, C.POINTER(C.c_uint64)]),
+    "dxmc_host_read_raw": (C.c_int, [C.c_char_p, C.c_void_p, C.c_uint64]),
+    "dxmc_host_write_raw": (C.c_int, [C.c_char_p, C.c_void_p, C.c_uint64]),
     "dxmc_host_world_extent": (C.c_int, [C.POINTER(HostWorld), f64p]),
     "dxmc_host_transport": (C.c_int, [C.POINTER(HostWorld), C.POINTER(SourceSpec), C.POINTER(HostTransportCfg), f64p, u32p, f64p,
                                       C.POINTER(RunStats), f64p, C.POINTER(C.c_uint64)]),

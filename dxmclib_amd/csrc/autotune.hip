@@ -38,7 +38,7 @@ struct Trial {
         double best = -1;
         for (int rep = 0; rep < 2; ++rep) {
             dxmc_run_stats s {};
-            status = runTransport(c, w, t, x.data(), x.size(), cfg, tally, nullptr, nullptr, &s);
+            status = runTransport(c, w, t, x.data(), x.size(), cfg, tally, nullptr, &s);
             if (status)
                 return -1;
             ++trials;

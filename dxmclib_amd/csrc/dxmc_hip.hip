@@ -162,6 +162,49 @@ __global__ void normalizeTally(Tally tally, uint64_t n, int mode, uint64_t nHist
     }
 }
 
+// Maximum of the dose along one grid axis for every pixel of the other two
+// (x fastest in the grid). Along y or z one thread per pixel walks the
+// axis, reading consecutive x across the wave; along x one wave per pixel
+// reduces the row.
+__global__ __launch_bounds__(256) void tallyMip(const double* __restrict__ dose, uint32_t nx, uint32_t ny, uint32_t nz, int axis, float* __restrict__ image)
+{
+    const uint64_t slice = uint64_t(nx) * ny;
+    if (axis == 0) {
+        const uint64_t pixel = (uint64_t(blockIdx.x) * blockDim.x + threadIdx.x) / 64;
+        const uint32_t lane = threadIdx.x % 64;
+        if (pixel >= uint64_t(ny) * nz)
+            return;
+        const double* row = dose + pixel * nx; // pixel = y + ny * z
+        double m = 0;
+        for (uint32_t x = lane; x < nx; x += 64)
+            m = fmax(m, row[x]);
+        for (int off = 32; off > 0; off >>= 1)
+            m = fmax(m, __shfl_xor(m, off));
+        if (lane == 0)
+            image[pixel] = static_cast<float>(m);
+        return;
+    }
+    const uint64_t pixel = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (axis == 1) { // pixel = x + nx * z
+        if (pixel >= uint64_t(nx) * nz)
+            return;
+        const uint64_t x = pixel % nx, z = pixel / nx;
+        const double* p = dose + z * slice + x;
+        double m = 0;
+        for (uint32_t y = 0; y < ny; ++y)
+            m = fmax(m, p[uint64_t(y) * nx]);
+        image[pixel] = static_cast<float>(m);
+        return;
+    }
+    if (pixel >= slice) // axis 2, pixel = x + nx * y
+        return;
+    const double* p = dose + pixel;
+    double m = 0;
+    for (uint32_t z = 0; z < nz; ++z)
+        m = fmax(m, p[uint64_t(z) * slice]);
+    image[pixel] = static_cast<float>(m);
+}
+
 __global__ void fillZero(double* a, uint32_t* b, double* c, uint64_t n)
 {
     const uint64_t i = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
@@ -421,6 +464,25 @@ int dxmc_hip_tally_download(dxmc_ctx* c, const dxmc_tally* t, double* dose, uint
     return DXMC_OK;
 }
 
+int dxmc_hip_tally_mip(dxmc_ctx* c, const dxmc_tally* t, const dxmc_world* w, int32_t axis, float* image)
+{
+    if (!c || !t || !w || !image || axis < 0 || axis > 2)
+        return c ? c->fail(DXMC_ERR_ARGUMENT, "tally_mip: NULL argument or axis not 0, 1 or 2") : DXMC_ERR_ARGUMENT;
+    if (t->ctx != c || w->ctx != c)
+        return c->fail(DXMC_ERR_STATE, "tally_mip: tally or world belongs to another context");
+    if (t->n != w->n)
+        return c->fail(DXMC_ERR_ARGUMENT, "tally_mip: tally and world sizes differ");
+    DXMC_CHECK(c, hipSetDevice(c->device));
+    const uint64_t pixels = w->n / w->dims[axis];
+    DeviceBuffer img(c);
+    uint64_t width = 0, height = 0;
+    if (int st = img.alloc(pixels * sizeof(float)))
+        return st;
+    if (int st = launchTallyMip(c, t, w, axis, static_cast<float*>(img.p), width, height))
+        return st;
+    return c->download(image, img.p, pixels * sizeof(float));
+}
+
 int dxmc_hip_tally_normalize(dxmc_ctx* c, dxmc_tally* t, const dxmc_world* w, int mode, uint64_t nHist, double calibration)
 {
     if (!c || !t || t->ctx != c)
@@ -444,6 +506,13 @@ int dxmc_hip_tally_normalize(dxmc_ctx* c, dxmc_tally* t, const dxmc_world* w, in
 
 int dxmc_hip_run(dxmc_ctx* c, const dxmc_world* w, const dxmc_tables* t, const dxmc_exposure* exposures, uint64_t nExp, const dxmc_run_cfg* cfg,
     dxmc_tally* tally, const volatile int32_t* cancel, volatile uint64_t* exposuresDone, dxmc_run_stats* stats)
+{
+    const dxmc_run_monitor monitor { cancel, exposuresDone, nullptr, nullptr, 0, 0.0f };
+    return dxmc_hip_run_monitored(c, w, t, exposures, nExp, cfg, tally, &monitor, stats);
+}
+
+int dxmc_hip_run_monitored(dxmc_ctx* c, const dxmc_world* w, const dxmc_tables* t, const dxmc_exposure* exposures, uint64_t nExp,
+    const dxmc_run_cfg* cfg, dxmc_tally* tally, const dxmc_run_monitor* monitor, dxmc_run_stats* stats)
 {
     if (!tally)
         return c ? c->fail(DXMC_ERR_ARGUMENT, "run: NULL argument") : DXMC_ERR_ARGUMENT;
@@ -471,7 +540,9 @@ int dxmc_hip_run(dxmc_ctx* c, const dxmc_world* w, const dxmc_tables* t, const d
             if (run.*f == 0)
                 run.*f = b.*f;
     }
-    return runTransport(c, w, t, exposures, nExp, run, tally, cancel, exposuresDone, stats);
+    if (monitor && monitor->preview && (monitor->preview_axis < 0 || monitor->preview_axis > 2))
+        return c->fail(DXMC_ERR_ARGUMENT, "run: preview axis must be 0, 1 or 2");
+    return runTransport(c, w, t, exposures, nExp, run, tally, monitor, stats);
 }
 
 } // extern "C"
@@ -636,9 +707,42 @@ int uploadExposures(dxmc_ctx* c, const dxmc_exposure* x, uint64_t nExp, RunBuffe
     return DXMC_OK;
 }
 
-int runTransport(dxmc_ctx* c, const dxmc_world* w, const dxmc_tables* t, const dxmc_exposure* x, uint64_t nExp, const dxmc_run_cfg& cfg,
-    dxmc_tally* tally, const volatile int32_t* cancel, volatile uint64_t* exposuresDone, dxmc_run_stats* stats)
+int launchTallyMip(dxmc_ctx* c, const dxmc_tally* tally, const dxmc_world* w, int axis, float* image, uint64_t& width, uint64_t& height)
 {
+    const uint32_t nx = w->dims[0], ny = w->dims[1], nz = w->dims[2];
+    width = axis == 0 ? ny : nx;
+    height = axis == 2 ? ny : nz;
+    const uint64_t threads = axis == 0 ? width * height * 64 : width * height;
+    hipLaunchKernelGGL(tallyMip, dim3(gridFor(threads, 256)), dim3(256), 0, c->stream, static_cast<const double*>(tally->dose), nx, ny, nz, axis,
+        image);
+    return c->check(hipGetLastError());
+}
+
+int runTransport(dxmc_ctx* c, const dxmc_world* w, const dxmc_tables* t, const dxmc_exposure* x, uint64_t nExp, const dxmc_run_cfg& cfg,
+    dxmc_tally* tally, const dxmc_run_monitor* monitor, dxmc_run_stats* stats)
+{
+    const volatile int32_t* cancel = monitor ? monitor->cancel : nullptr;
+    volatile uint64_t* exposuresDone = monitor ? monitor->exposures_done : nullptr;
+    const bool preview = monitor && monitor->preview;
+    DeviceBuffer image(c);
+    std::vector<float> hostImage;
+    auto lastPreview = std::chrono::steady_clock::now();
+    // the live preview: projection of the tally so far, to the host, to the callback
+    auto sendPreview = [&]() -> int {
+        uint64_t width = 0, height = 0;
+        const uint64_t pixels = uint64_t(w->dims[0]) * w->dims[1] * w->dims[2] / w->dims[monitor->preview_axis];
+        if (!image.p)
+            if (int e = image.alloc(pixels * sizeof(float)))
+                return e;
+        hostImage.resize(pixels);
+        if (int e = launchTallyMip(c, tally, w, monitor->preview_axis, static_cast<float*>(image.p), width, height))
+            return e;
+        if (int e = c->download(hostImage.data(), image.p, pixels * sizeof(float)))
+            return e;
+        monitor->preview(monitor->preview_user, hostImage.data(), width, height);
+        lastPreview = std::chrono::steady_clock::now();
+        return DXMC_OK;
+    };
     DXMC_CHECK(c, hipSetDevice(c->device));
     RunBuffers rb(c);
     int st = uploadExposures(c, x, nExp, rb);
@@ -685,6 +789,11 @@ int runTransport(dxmc_ctx* c, const dxmc_world* w, const dxmc_tables* t, const d
             ++done;
         if (exposuresDone)
             *exposuresDone = done;
+        const bool last = he >= total;
+        if (preview
+            && (last || std::chrono::duration<double>(std::chrono::steady_clock::now() - lastPreview).count() >= monitor->preview_interval_seconds))
+            if (int e = sendPreview())
+                return e;
     }
     if (stats) {
         Counters h {};

@@ -331,7 +331,10 @@ int autotune(dxmc_ctx* c, const dxmc_world* w, const dxmc_tables* t, const dxmc_
 // resident blocks per CU of the FP32 kernel (transport_fp32.hip)
 uint32_t residentBlocksFp32(int L, uint32_t wavesPerSimd, uint32_t ldsPerBlock);
 int runTransport(dxmc_ctx* c, const dxmc_world* w, const dxmc_tables* t, const dxmc_exposure* x, uint64_t nExp, const dxmc_run_cfg& cfg,
-    dxmc_tally* tally, const volatile int32_t* cancel, volatile uint64_t* exposuresDone, dxmc_run_stats* stats);
+    dxmc_tally* tally, const dxmc_run_monitor* monitor, dxmc_run_stats* stats);
+// maximum intensity projection of the tally's dose along axis into a device
+// image (launch only, on the context's stream); width and height returned
+int launchTallyMip(dxmc_ctx* c, const dxmc_tally* tally, const dxmc_world* w, int axis, float* image, uint64_t& width, uint64_t& height);
 // FP32 transport launch (transport_fp32.hip)
 void launchFp32(dxmc_ctx* c, const dxmc_run_cfg& cfg, const dxmc_dev::World<float>& w, const dxmc_dev::Tables<float>& t,
     const dxmc_dev::Exposure<float>* exps, const uint64_t* expStart, uint32_t nExp, uint64_t hb, uint64_t he, const dxmc_dev::Tally& tally,

@@ -312,9 +312,18 @@ protected:
             const dxmc_tables_desc td = tables.desc();
             ok = !(dxmc_hip_tables_upload(c, &td, &dt) || dxmc_hip_tally_create(c, world.size(), &tally));
             t1 = clock::now();
-            ok = ok
-                && !dxmc_hip_run(c, dw, dt, prep.exposures.data(), prep.exposures.size(), &cfg, tally, pb ? pb->cancelFlag() : nullptr,
-                    pb ? pb->exposureCounter() : nullptr, &m_stats);
+            // progress, cancel and the live dose preview go through the
+            // progress bar, as the reference's workers report to it
+            dxmc_run_monitor monitor {};
+            if (pb) {
+                monitor.cancel = pb->cancelFlag();
+                monitor.exposures_done = pb->exposureCounter();
+                monitor.preview = &ProgressBar<T>::previewCallback;
+                monitor.preview_user = pb;
+                monitor.preview_axis = pb->planeNormal() == ProgressBar<T>::Axis::X ? 0 : pb->planeNormal() == ProgressBar<T>::Axis::Y ? 1 : 2;
+                monitor.preview_interval_seconds = static_cast<float>(pb->previewInterval());
+            }
+            ok = ok && !dxmc_hip_run_monitored(c, dw, dt, prep.exposures.data(), prep.exposures.size(), &cfg, tally, &monitor, &m_stats);
         }
         if (ok && !(pb && pb->cancel())) {
             ok = !(nHist > 1 && dxmc_hip_tally_normalize(c, tally, dw, mode, nHist, mode == DXMC_OUTPUT_DOSE ? calibration() : 1.0))

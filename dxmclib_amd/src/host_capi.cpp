@@ -4,6 +4,7 @@
 
 #include "dxmc/attenuationlut.hpp"
 #include "dxmc/devicetables.hpp"
+#include "dxmc/rawio.hpp"
 #include "dxmc/source.hpp"
 #include "dxmc/transport.hpp"
 #include "dxmc/world.hpp"
@@ -320,6 +321,27 @@ int dxmc_host_source_spectrum(const dxmc_source_spec* spec, double* energies, do
         energies[i] = s[i].first;
         weights[i] = s[i].second;
     }
+    return DXMC_OK;
+}
+
+int dxmc_host_read_raw(const char* path, void* data, uint64_t bytes)
+{
+    if (!path || (!data && bytes))
+        return fail(DXMC_ERR_ARGUMENT, "read_raw: NULL argument");
+    const auto a = readBinaryArray<unsigned char>(path, bytes);
+    if (!a)
+        return fail(DXMC_ERR_ARGUMENT, std::string("read_raw: ") + path + " is missing or does not hold exactly " + std::to_string(bytes) + " bytes");
+    std::copy(a->cbegin(), a->cend(), static_cast<unsigned char*>(data));
+    return DXMC_OK;
+}
+
+int dxmc_host_write_raw(const char* path, const void* data, uint64_t bytes)
+{
+    if (!path || (!data && bytes))
+        return fail(DXMC_ERR_ARGUMENT, "write_raw: NULL argument");
+    const auto* p = static_cast<const unsigned char*>(data);
+    if (!saveBinaryArray(std::vector<unsigned char>(p, p + bytes), path))
+        return fail(DXMC_ERR_ARGUMENT, std::string("write_raw: cannot write ") + path);
     return DXMC_OK;
 }
 

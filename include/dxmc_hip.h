@@ -179,6 +179,11 @@ int dxmc_hip_tally_destroy(dxmc_ctx* ctx, dxmc_tally* tally);
 int dxmc_hip_tally_reset(dxmc_ctx* ctx, dxmc_tally* tally);
 /* Copy to host; any pointer may be NULL to skip that array. */
 int dxmc_hip_tally_download(dxmc_ctx* ctx, const dxmc_tally* tally, double* dose, uint32_t* n_events, double* variance);
+/* Maximum intensity projection of a tally's dose array along axis (0 x,
+ * 1 y, 2 z) of the world's grid, laid out as the run preview of
+ * dxmc_hip_run_monitored; image holds the product of the other two
+ * dimensions. */
+int dxmc_hip_tally_mip(dxmc_ctx* ctx, const dxmc_tally* tally, const dxmc_world* world, int32_t axis, float* image);
 /* In-place FP64 normalisation on the device:
  * EV_PER_HISTORY: normalizeScoring (transport.hpp:780-794),
  * DOSE: energyImpartedToDose with a calibration factor (transport.hpp:796-816). */
@@ -223,6 +228,29 @@ typedef struct {
 int dxmc_hip_run(dxmc_ctx* ctx, const dxmc_world* world, const dxmc_tables* tables, const dxmc_exposure* exposures,
     uint64_t n_exposures, const dxmc_run_cfg* cfg, dxmc_tally* tally, const volatile int32_t* cancel,
     volatile uint64_t* exposures_done, dxmc_run_stats* stats);
+
+/* Live view of a running transport: the dose preview of the reference's
+ * ProgressBar::computeDoseProgressImage (progressbar.hpp:116-235), computed
+ * on the device. 'preview' is called on the thread running
+ * dxmc_hip_run_monitored, between kernel launches, at most every
+ * preview_interval_seconds and once after the last launch, with the maximum
+ * intensity projection of the tally's accumulated dose array along
+ * preview_axis (0 x, 1 y, 2 z): width x height floats, first index fastest,
+ * over (y, z) for x, (x, z) for y and (x, y) for z (dxmc_hip_tally_mip). */
+typedef void (*dxmc_preview_fn)(void* user, const float* image, uint64_t width, uint64_t height);
+typedef struct {
+    const volatile int32_t* cancel; /* as dxmc_hip_run; may be NULL */
+    volatile uint64_t* exposures_done; /* as dxmc_hip_run; may be NULL */
+    dxmc_preview_fn preview; /* NULL: no preview */
+    void* preview_user;
+    int32_t preview_axis;
+    float preview_interval_seconds;
+} dxmc_run_monitor;
+
+/* dxmc_hip_run with a monitor (cancel, progress, live preview); monitor may
+ * be NULL. */
+int dxmc_hip_run_monitored(dxmc_ctx* ctx, const dxmc_world* world, const dxmc_tables* tables, const dxmc_exposure* exposures,
+    uint64_t n_exposures, const dxmc_run_cfg* cfg, dxmc_tally* tally, const dxmc_run_monitor* monitor, dxmc_run_stats* stats);
 
 /* Autotuner of the transport kernels (no counterpart in the reference, whose
  * CPU threads need no tuning). Times trial launches of trial_histories

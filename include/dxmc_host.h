@@ -99,6 +99,13 @@ typedef struct {
     const char* const* material_names;
 } dxmc_host_world;
 
+/* Raw binary arrays (no header, host byte order), as the reference's
+ * validation program keeps phantoms and dose maps (validation.cpp:128-135,
+ * 1280-1306). dxmc_host_read_raw fails unless the file holds exactly
+ * 'bytes' bytes. */
+int dxmc_host_read_raw(const char* path, void* data, uint64_t bytes);
+int dxmc_host_write_raw(const char* path, const void* data, uint64_t bytes);
+
 /* The nextafter-shrunk extents of a world (World::matrixExtentSafe). */
 int dxmc_host_world_extent(const dxmc_host_world* w, double extent_safe[6]);
 

@@ -332,6 +332,50 @@ def test_bounded_density_codes_exact(device, n_materials):
     assert np.allclose(out[0][2], out[1][2], rtol=1e-10, atol=0)
 
 
+@pytest.mark.parametrize("axis", [0, 1, 2])
+def test_live_preview_and_mip(device, axis):
+    """The run loop's live dose preview (ProgressBar::computeDoseProgressImage,
+    progressbar.hpp:116-235, computed here on the device): after every launch
+    the maximum intensity projection of the tally so far; previews never
+    decrease, the last equals the projection of the final tally, and
+    dxmc_hip_tally_mip projects like numpy along every axis."""
+    print(f"live preview axis {axis}", flush=True)
+    rng = np.random.default_rng(12)
+    shape = (40, 36, 32)  # z, y, x: distinct sizes catch a transposed image
+    nz, ny, nx = shape
+    mat = (rng.random(shape) < 0.3).astype(np.uint8)
+    dens = np.array([1.0, 1.92], dtype=np.float32)[mat]
+    spacing = (1.0, 1.0, 1.0)
+    ext = backend.world_extent((nx, ny, nz), spacing)
+    t = backend.HostTables(["Water, Liquid", "Bone, Cortical (ICRU)"], [1.0, 1.92], 120.0, 1.0)
+    spec = backend.source_spec(capi.SOURCE_DX, tube_voltage=90.0, al_mm=2.5, sdd=1000.0, field_size=(30.0, 30.0),
+                               histories_per_exposure=25_000, total_exposures=8)
+    exps = t.exposures(spec)
+    w = device.upload_world(mat, dens, spacing, ext, 2)
+    tt = device.upload_tables(t.desc())
+    tally = device.tally(mat.size)
+    try:
+        cfg = capi.RunCfg()
+        cfg.histories_per_launch = 50_000
+        images = []
+        stats = device.run_monitored(w, tt, exps, tally, capi.PRECISION_FP32, 1, seed=42, tuning=cfg, preview=images.append,
+                                     preview_axis=axis, preview_interval=0.0)
+        assert stats.launches == 4 and len(images) == 4
+        d, _, _ = device.download(tally, mat.size)
+        vol = d.reshape(shape)
+        want = vol.max(axis=2 - axis).astype(np.float32)
+        assert images[-1].shape == want.shape and np.array_equal(images[-1], want)
+        for a, b in zip(images, images[1:]):
+            assert np.all(b >= a)
+        assert images[0].max() > 0
+        for ax in (0, 1, 2):
+            assert np.array_equal(device.tally_mip(tally, w, (nx, ny, nz), ax), vol.max(axis=2 - ax).astype(np.float32))
+    finally:
+        device.destroy_tally(tally)
+        device.destroy_tables(tt)
+        device.destroy_world(w)
+
+
 @pytest.mark.parametrize("dtype", [np.float32, np.float64])
 def test_world_scan_on_device(device, dtype):
     """The upload's passes over the voxels run on the device: each material's
