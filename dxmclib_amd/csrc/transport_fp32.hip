@@ -421,15 +421,14 @@ uint32_t stageLevelFp32(const dxmc_run_cfg& cfg, const Tables<float>& t)
             --level;
         return level;
     }
-    // all tables while at least 3 blocks (12 waves) stay resident per CU:
-    // the attenuation coefficients in LDS are worth more than the 4th and
-    // 5th block (measured on the CT workload)
-    const uint32_t blocks = std::min<uint32_t>(3, residentBlocksFp32(cfg.low_energy_correction, cfg.waves_per_simd, 0));
-    constexpr uint32_t ldsPerCU = 160 * 1024;
-    if (fits(STAGE_ALL) && bytes(STAGE_ALL) * blocks <= ldsPerCU)
-        return STAGE_ALL;
-    if (fits(STAGE_SMALL))
-        return STAGE_SMALL;
+    // the most tables that keep the occupancy the registers allow: a
+    // resident block is worth more than the attenuation coefficients in LDS
+    // (CT spiral 2.84e9 vs 2.50e9, DX 4.07e9 vs 3.32e9 histories/s with
+    // 5 and 3 blocks per CU)
+    const uint32_t best = residentBlocksFp32(cfg.low_energy_correction, cfg.waves_per_simd, bytes(STAGE_NONE));
+    for (uint32_t level = STAGE_ALL; level > STAGE_NONE; --level)
+        if (fits(level) && residentBlocksFp32(cfg.low_energy_correction, cfg.waves_per_simd, static_cast<uint32_t>(bytes(level))) >= best)
+            return level;
     return STAGE_NONE;
 }
 
