@@ -27,7 +27,7 @@ OUTPUT_EV_PER_HISTORY = 0
 OUTPUT_DOSE = 1
 RUN_COUNTERS = 1
 RUN_AUTOTUNE = 2
-RUN_FULL_RECORDS = 4
+RUN_EXACT_STEPS = 4
 
 SOURCE_PENCIL = 0
 SOURCE_ISOTROPIC = 1

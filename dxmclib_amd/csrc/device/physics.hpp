@@ -89,6 +89,9 @@ struct Tables {
     const float* lutAnchor; // FP32: log2 E of the nodes
     float lutLinearEnergy2, lutLinearStepInv2; // FP32: linear index start and 1/step, log2 units
     const T* lutCoef; // [mat][node][type][b,a]
+    // FP32: [mat][node] lower and upper bound of the total attenuation the
+    // FP32 evaluation gives at any energy its segment serves (null: none)
+    const float2* lutSumBound;
     const T* lutMajorant; // [node][b,a]
     const T* rita; // [mat][x,e,a,b][56]
     const T* spline; // [mat][60 + start, step, stop]

@@ -115,6 +115,58 @@ namespace {
         bound[k] = __longlong_as_double(static_cast<long long>(hi));
     }
 
+    // Bounds of the FP32 total attenuation (photo + Compton + Rayleigh) of
+    // each material over each LUT segment: the energies the FP64 segment
+    // search sends to segment k are [bound[k], bound[k+1]); each component
+    // 2^(c0 + c1 (log2 E - anchor_k)) is monotone there, so its extremes sit
+    // at the range's ends (infinite ends give 0 or infinity). The sums are
+    // widened by a relative margin of 2e-5 + 4e-6 sum|c1|, ten times the
+    // error of the device's evaluation (hardware log2 / exp2 within a few
+    // ulp, float coefficients), then rounded outward to float. A segment no
+    // energy reaches gets an empty interval (lo > hi).
+    std::vector<float2> totalAttenuationBounds(const std::vector<float>& coef, const std::vector<float>& anchor, const std::vector<double>& bound,
+        uint32_t nMat, uint32_t nodes)
+    {
+        auto below = [](double v) {
+            float f = static_cast<float>(v);
+            return static_cast<double>(f) > v ? std::nextafter(f, -HUGE_VALF) : f;
+        };
+        auto above = [](double v) {
+            float f = static_cast<float>(v);
+            return static_cast<double>(f) < v ? std::nextafter(f, HUGE_VALF) : f;
+        };
+        auto component = [](double c0, double c1, double anchorK, double E) {
+            if (E <= 0)
+                return c1 > 0 ? 0.0 : (c1 < 0 ? HUGE_VAL : std::exp2(c0));
+            if (std::isinf(E))
+                return c1 < 0 ? 0.0 : (c1 > 0 ? HUGE_VAL : std::exp2(c0));
+            return std::exp2(c0 + c1 * (std::log2(E) - anchorK));
+        };
+        std::vector<float2> out(size_t(nMat) * nodes);
+        for (uint32_t m = 0; m < nMat; ++m)
+            for (uint32_t k = 0; k < nodes; ++k) {
+                const double eLo = k == 0 ? 0.0 : bound[k];
+                const double eHi = k + 1 < nodes ? bound[k + 1] : HUGE_VAL;
+                float2& r = out[size_t(m) * nodes + k];
+                if (!(eLo < eHi)) {
+                    r = make_float2(HUGE_VALF, 0.0f);
+                    continue;
+                }
+                const float* c = &coef[(size_t(m) * nodes + k) * 6];
+                double lo = 0, hi = 0, slope = 0;
+                for (int i = 0; i < 3; ++i) {
+                    const double a = component(c[2 * i], c[2 * i + 1], anchor[k], eLo);
+                    const double b = component(c[2 * i], c[2 * i + 1], anchor[k], eHi);
+                    lo += std::min(a, b);
+                    hi += std::max(a, b);
+                    slope += std::fabs(c[2 * i + 1]);
+                }
+                const double margin = 2e-5 + 4e-6 * slope;
+                r = make_float2(below(lo * (1 - margin)), above(hi * (1 + margin)));
+            }
+        return out;
+    }
+
     template <typename T, typename S>
     int upload(dxmc_ctx* c, dxmc_tables& t, const std::vector<S>& host, const T** dst)
     {
@@ -180,6 +232,16 @@ namespace {
             pair(d.lut_coefficients + i, (i / 6) % d.lut_nodes, &coef[i]);
         if ((st = upload(c, t, coef, &out.lutCoef)))
             return st;
+        out.lutSumBound = nullptr;
+        if constexpr (std::is_same_v<T, float>) {
+            std::vector<double> bounds(d.lut_nodes);
+            if ((st = c->download(bounds.data(), out.lutBound, d.lut_nodes * sizeof(double))))
+                return st;
+            const auto anchors = cast<float>(d.lut_grid, d.lut_nodes, LOG2_10);
+            const auto sums = totalAttenuationBounds(coef, anchors, bounds, d.n_materials, d.lut_nodes);
+            if ((st = upload(c, t, sums, &out.lutSumBound)))
+                return st;
+        }
         std::vector<T> maj(size_t(d.lut_nodes) * 2);
         for (size_t i = 0; i < maj.size(); i += 2)
             pair(d.lut_majorant + i, i / 2, &maj[i]);

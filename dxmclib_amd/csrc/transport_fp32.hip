@@ -63,7 +63,8 @@ __device__ inline uint64_t voxelIndexFast(const World<float>& w, const float* po
 // into LDS at kernel start, one copy per block: their lookups then cost LDS
 // latency instead of L2 latency, also while the world gathers keep evicting
 // them from L1. Staging level: 1 none, 2 all but the attenuation
-// coefficients, 3 all.
+// coefficients, 3 all. The total-attenuation bounds stay in global memory:
+// in LDS they cost a resident block per CU.
 enum : uint32_t { STAGE_NONE = 1, STAGE_SMALL = 2, STAGE_ALL = 3 };
 
 __host__ __device__ constexpr uint32_t ldsAlign(uint64_t bytes) { return static_cast<uint32_t>((bytes + 15) & ~uint64_t(15)); }
@@ -322,29 +323,41 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(W, 8))) voi
             idx = voxelIndexFast(w, p.pos, inv); // clamped: a valid voxel also outside
             Rng r2 = rng;
             const float u2 = r2.uniform<float>();
-            float att[3];
+            // The acceptance probability (Σμ·ρ)·(1/μ̂) is monotone in Σμ and
+            // in ρ. Bounds of both settle most draws exactly as the values
+            // would: u2 < p(lower bounds) accepts, u2 >= p(upper bounds)
+            // rejects. Σμ is bounded over the photon's LUT segment; ρ by the
+            // voxel's code when the world has codes, else read exactly. Only
+            // a draw between the two evaluates the attenuation (and reads the
+            // record).
             float prob;
+            float dLo, dHi;
             if (w.codeBytes != 0) {
-                // the voxel's code gives its material and lo <= density <= hi;
-                // the acceptance probability is monotone in the density, so
-                // u2 < p(lo) accepts and u2 >= p(hi) rejects exactly as the
-                // density itself would; in between, the record decides
                 const uint32_t code = w.codeBytes == 1 ? static_cast<const uint8_t*>(w.codes)[idx] : static_cast<const uint16_t*>(w.codes)[idx];
                 const DensityBound b = w.bounds[code];
                 mat = b.material;
                 meas = 0;
-                attenuationAt(t, mat, lp, att);
-                const float sum = (att[0] + att[1]) + att[2];
-                const float lo = sum * b.lo * majInv;
-                const float hi = sum * b.hi * majInv;
+                dLo = b.lo;
+                dHi = b.hi;
                 density = b.lo;
-                prob = lo;
-                if (u2 >= lo && u2 < hi) {
-                    density = __uint_as_float(static_cast<const uint2*>(w.voxels)[idx].x);
-                    prob = sum * density * majInv;
-                }
             } else {
                 readVoxel(w, idx, mat, density, meas);
+                dLo = density;
+                dHi = density;
+            }
+            bool settled = false;
+            prob = 0.0f;
+            if (t.lutSumBound) {
+                const float2 sb = t.lutSumBound[static_cast<size_t>(mat) * t.lutNodes + lp.k];
+                const float lo = sb.x * dLo * majInv;
+                const float hi = sb.y * dHi * majInv;
+                settled = (u2 < lo) | (u2 >= hi);
+                prob = lo;
+            }
+            if (!settled) {
+                if (w.codeBytes != 0)
+                    density = __uint_as_float(static_cast<const uint2*>(w.voxels)[idx].x);
+                float att[3];
                 attenuationAt(t, mat, lp, att);
                 prob = ((att[0] + att[1]) + att[2]) * density * majInv;
             }
@@ -439,8 +452,17 @@ void launchFp32(dxmc_ctx* c, const dxmc_run_cfg& cfg, const World<float>& w, con
     hipMemsetAsync(nx, 0, sizeof(unsigned long long), c->stream);
     const uint32_t block = 256;
     const uint32_t waves = cfg.waves_per_simd;
-    const uint32_t stage = stageLevelFp32(cfg, t);
-    const uint32_t lds = ldsBytes(t, stage) + WAVES_PER_BLOCK * QUEUE_BYTES;
+    World<float> world = w;
+    Tables<float> tables = t;
+    // the attenuation bounds pay where they also spare the record read
+    // (worlds with codes); elsewhere the extra lookup costs more than the
+    // evaluations it skips in diverged waves
+    if ((cfg.flags & DXMC_RUN_EXACT_STEPS) || world.codeBytes == 0) {
+        world.codeBytes = 0;
+        tables.lutSumBound = nullptr;
+    }
+    const uint32_t stage = stageLevelFp32(cfg, tables);
+    const uint32_t lds = ldsBytes(tables, stage) + WAVES_PER_BLOCK * QUEUE_BYTES;
     // default: as many blocks as stay resident on a CU (registers and LDS)
     const uint32_t perCU = cfg.blocks_per_cu ? cfg.blocks_per_cu : residentBlocksFp32(cfg.low_energy_correction, waves, lds);
     const uint32_t regen = cfg.regeneration_threshold ? std::min<uint32_t>(cfg.regeneration_threshold, 64u) : 4u;
@@ -450,10 +472,7 @@ void launchFp32(dxmc_ctx* c, const dxmc_run_cfg& cfg, const World<float>& w, con
     uint64_t grid = static_cast<uint64_t>(c->numCU) * perCU;
     const uint64_t lanesNeeded = (he - hb + 63) / 64 * 64;
     grid = std::max<uint64_t>(1, std::min<uint64_t>(grid, (lanesNeeded + block - 1) / block));
-    World<float> world = w;
-    if (cfg.flags & DXMC_RUN_FULL_RECORDS)
-        world.codeBytes = 0;
-    hipLaunchKernelGGL(kernelFor(cfg.low_energy_correction, waves, stage), dim3(static_cast<uint32_t>(grid)), dim3(block), lds, c->stream, world, t,
+    hipLaunchKernelGGL(kernelFor(cfg.low_energy_correction, waves, stage), dim3(static_cast<uint32_t>(grid)), dim3(block), lds, c->stream, world, tables,
         exps, expStart, nExp, hb, he, cfg.seed, cfg.first_history, tally, ctr, nx, regen, interact, pool);
 }
 }

@@ -210,7 +210,7 @@ typedef struct {
 /* dxmc_run_cfg.flags */
 #define DXMC_RUN_COUNTERS 1u /* collect step/event counters */
 #define DXMC_RUN_AUTOTUNE 2u /* fill zero tuning fields from the autotuner (tuned once per world/tables/precision/L and kept in the context) */
-#define DXMC_RUN_FULL_RECORDS 4u /* FP32: read every step's full voxel record instead of deciding from the bounded-density codes (same results, for checks) */
+#define DXMC_RUN_EXACT_STEPS 4u /* FP32: evaluate every Woodcock step from the voxel record and the full attenuation, without the bounds that settle most steps (same results, for checks) */
 
 typedef struct {
     uint64_t histories; /* histories started */

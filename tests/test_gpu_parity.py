@@ -285,27 +285,32 @@ def test_fp32_voxel_records(device, records):
     assert abs(s32.events - s64.events) <= 1e-3 * s64.events
 
 
-@pytest.mark.parametrize("n_materials", [3, 70])
-def test_bounded_density_codes_exact(device, n_materials):
-    """A world with 8-byte voxel records also gets a 1-byte (up to 64
-    materials) or 2-byte code per voxel: its material and an interval that
-    holds its density. The FP32 kernel settles each Woodcock decision from
-    the interval where it can and reads the record otherwise; the transport
-    is identical to reading every record (DXMC_RUN_FULL_RECORDS)."""
-    print(f"bounded density codes, {n_materials} materials", flush=True)
+@pytest.mark.parametrize("n_materials,jitter", [(3, True), (70, True), (3, False)])
+def test_bounded_steps_exact(device, n_materials, jitter):
+    """The FP32 kernel settles most Woodcock decisions from bounds: the total
+    attenuation bounded over the photon's LUT segment, and the density
+    bounded by a 1-byte (up to 64 materials) or 2-byte code per voxel in
+    worlds with 8-byte records (material and an interval holding the
+    density), or read exactly from the palette. Only draws between the
+    bounds evaluate the attenuation and read the record. The transport is
+    identical to evaluating every step in full (DXMC_RUN_EXACT_STEPS)."""
+    print(f"bounded steps, {n_materials} materials, jitter {jitter}", flush=True)
     rng = np.random.default_rng(8)
     shape = (48, 48, 48)
     names = (NAMES * 20)[:n_materials]
     dens_of = np.array((MAXD * 20)[:n_materials], dtype=np.float32)
     dens_of[dens_of < 0.01] = 0.5  # no near-vacuum: every material interacts
     mat = rng.integers(0, n_materials, shape).astype(np.uint8)
-    dens = (dens_of[mat] * rng.uniform(0.9, 1.1, shape)).astype(np.float32)
+    dens = (dens_of[mat] * (rng.uniform(0.9, 1.1, shape) if jitter else 1.0)).astype(np.float32)
     spacing = (1.0, 1.0, 1.0)
     ext = backend.world_extent(shape[::-1], spacing)
     w = device.upload_world(mat, dens, spacing, ext, n_materials)
     code_bytes = 1 if n_materials <= 64 else 2
     n = mat.size
-    assert device.world_bytes(w) == n * (1 + 8 + 8 + code_bytes) + 16 * (256 if code_bytes == 1 else 65536)
+    if jitter:
+        assert device.world_bytes(w) == n * (1 + 8 + 8 + code_bytes) + 16 * (256 if code_bytes == 1 else 65536)
+    else:
+        assert device.world_bytes(w) == n * (1 + 8 + 1) + 8 * n_materials
     maxd = device.world_max_density(w, n_materials)
     t = backend.HostTables(names, maxd, 120.0, 1.0)
     spec = backend.source_spec(capi.SOURCE_DX, tube_voltage=120.0, al_mm=2.5, sdd=1000.0, field_size=(48.0, 48.0),
@@ -314,7 +319,7 @@ def test_bounded_density_codes_exact(device, n_materials):
     tt = device.upload_tables(t.desc())
     out = []
     try:
-        for flags in (capi.RUN_COUNTERS, capi.RUN_COUNTERS | capi.RUN_FULL_RECORDS):
+        for flags in (capi.RUN_COUNTERS, capi.RUN_COUNTERS | capi.RUN_EXACT_STEPS):
             cfg = capi.RunCfg()
             cfg.flags = flags
             tally = device.tally(n)
