@@ -244,7 +244,7 @@ def main():
         tune_report = None
         if not args.no_autotune:
             t1 = time.perf_counter()
-            tuning, rep = dev.autotune(w, t, exposures, capi.PRECISION_FP32, L, trial_histories=1 << 23, base=tuning)
+            tuning, rep = dev.autotune(w, t, exposures, capi.PRECISION_FP32, L, trial_histories=1 << 25, base=tuning)
             tuning.flags = 1
             tune_report = {"seconds": time.perf_counter() - t1, "trials": rep.trials, "default_rate": rep.default_rate,
                            "best_rate": rep.best_rate, "regeneration_threshold": tuning.regeneration_threshold,

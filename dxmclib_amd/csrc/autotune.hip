@@ -49,6 +49,11 @@ struct Trial {
     }
 };
 
+// A value replaces the incumbent only when it is faster by more than this
+// fraction: trial timings scatter by about a percent, and taking every
+// lucky measurement drifts away from the defaults for nothing.
+constexpr double MIN_GAIN = 0.015;
+
 // coordinate search over one field: keeps the value with the best rate
 int searchField(Trial& tr, dxmc_run_cfg& cfg, uint32_t dxmc_run_cfg::*field, const std::vector<uint32_t>& values, double& bestRate)
 {
@@ -62,7 +67,7 @@ int searchField(Trial& tr, dxmc_run_cfg& cfg, uint32_t dxmc_run_cfg::*field, con
         const double r = tr.rate(k, st);
         if (st)
             return st;
-        if (r > bestRate) {
+        if (r > bestRate * (1 + MIN_GAIN)) {
             bestRate = r;
             bestValue = v;
         }
