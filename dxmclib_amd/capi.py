@@ -293,6 +293,8 @@ HIP_SYMBOLS = {
     "dxmc_hip_world_upload": (C.c_int, [C.c_void_p, C.POINTER(WorldDesc), C.POINTER(C.c_void_p)]),
     "dxmc_hip_world_destroy": (C.c_int, [C.c_void_p, C.c_void_p]),
     "dxmc_hip_world_bytes": (C.c_int, [C.c_void_p, C.POINTER(C.c_uint64)]),
+    "dxmc_hip_host_alloc": (C.c_int, [C.c_uint64, C.POINTER(C.c_void_p)]),
+    "dxmc_hip_host_free": (C.c_int, [C.c_void_p]),
     "dxmc_hip_world_max_density": (C.c_int, [C.c_void_p, C.POINTER(C.c_double), C.c_uint32]),
     "dxmc_hip_tables_upload": (C.c_int, [C.c_void_p, C.POINTER(TablesDesc), C.POINTER(C.c_void_p)]),
     "dxmc_hip_tables_destroy": (C.c_int, [C.c_void_p, C.c_void_p]),

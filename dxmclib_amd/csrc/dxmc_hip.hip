@@ -350,6 +350,30 @@ int dxmc_hip_mem_info(dxmc_ctx* c, uint64_t* freeBytes, uint64_t* totalBytes)
     return DXMC_OK;
 }
 
+int dxmc_hip_host_alloc(uint64_t bytes, void** out)
+{
+    if (!out || bytes == 0)
+        return setGlobalError("host_alloc: NULL argument or zero size"), DXMC_ERR_ARGUMENT;
+    *out = nullptr;
+    if (hipHostMalloc(out, bytes, hipHostMallocDefault) != hipSuccess) {
+        hipGetLastError();
+        *out = nullptr;
+        return setGlobalError("host_alloc: page-locked allocation of " + std::to_string(bytes) + " bytes failed"), DXMC_ERR_MEMORY;
+    }
+    registerPinned(*out, bytes);
+    return DXMC_OK;
+}
+
+int dxmc_hip_host_free(void* p)
+{
+    if (!p)
+        return DXMC_OK;
+    if (!unregisterPinned(p))
+        return setGlobalError("host_free: not from dxmc_hip_host_alloc"), DXMC_ERR_ARGUMENT;
+    hipHostFree(p);
+    return DXMC_OK;
+}
+
 int dxmc_hip_synchronize(dxmc_ctx* c)
 {
     if (!c)

@@ -14,6 +14,14 @@
 #include <type_traits>
 #include <vector>
 
+namespace dxmc_backend {
+// page-locked blocks from dxmc_hip_host_alloc; pinnedHost: whether
+// [p, p + bytes) lies in one of them
+void registerPinned(const void* p, size_t bytes);
+bool unregisterPinned(const void* p);
+bool pinnedHost(const void* p, size_t bytes);
+}
+
 struct dxmc_ctx {
     int device = 0;
     int numCU = 256;
@@ -98,7 +106,7 @@ struct dxmc_ctx {
     {
         if (bytes == 0)
             return DXMC_OK;
-        if (bytes < STAGE_MIN || !staging()) {
+        if (bytes < STAGE_MIN || dxmc_backend::pinnedHost(src, bytes) || !staging()) {
             if (int st = check(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, stream)))
                 return st;
             return check(hipStreamSynchronize(stream));
@@ -125,7 +133,7 @@ struct dxmc_ctx {
     {
         if (bytes == 0)
             return DXMC_OK;
-        if (bytes < STAGE_MIN || !staging()) {
+        if (bytes < STAGE_MIN || dxmc_backend::pinnedHost(dst, bytes) || !staging()) {
             if (int st = check(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, stream)))
                 return st;
             return check(hipStreamSynchronize(stream));

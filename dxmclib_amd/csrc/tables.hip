@@ -11,6 +11,7 @@
 #include "internal.hpp"
 
 #include <cmath>
+#include <map>
 #include <mutex>
 
 namespace dxmc_backend {
@@ -30,6 +31,32 @@ const char* globalError()
 {
     std::lock_guard lock(gErrMutex);
     return gErr.c_str();
+}
+
+// page-locked blocks from dxmc_hip_host_alloc: start -> size
+namespace {
+    std::mutex gPinnedMutex;
+    std::map<uintptr_t, size_t> gPinned;
+}
+void registerPinned(const void* p, size_t bytes)
+{
+    std::lock_guard lock(gPinnedMutex);
+    gPinned[reinterpret_cast<uintptr_t>(p)] = bytes;
+}
+bool unregisterPinned(const void* p)
+{
+    std::lock_guard lock(gPinnedMutex);
+    return gPinned.erase(reinterpret_cast<uintptr_t>(p)) != 0;
+}
+bool pinnedHost(const void* p, size_t bytes)
+{
+    const auto a = reinterpret_cast<uintptr_t>(p);
+    std::lock_guard lock(gPinnedMutex);
+    auto it = gPinned.upper_bound(a);
+    if (it == gPinned.begin())
+        return false;
+    --it;
+    return a >= it->first && a + bytes <= it->first + it->second;
 }
 
 bool validateTables(const dxmc_tables_desc& d, std::string& why)

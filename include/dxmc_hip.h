@@ -66,6 +66,12 @@ int dxmc_hip_abi_version(void);
 int dxmc_hip_mem_info(dxmc_ctx* ctx, uint64_t* free_bytes, uint64_t* total_bytes);
 /* Wait for all work of the context. */
 int dxmc_hip_synchronize(dxmc_ctx* ctx);
+/* Page-locked host memory (no counterpart in the reference, whose results
+ * never leave host memory). Copies between it and the device run at the
+ * link rate without staging through a bounce buffer; a caller that reuses
+ * result arrays (a request server) allocates them here once. */
+int dxmc_hip_host_alloc(uint64_t bytes, void** out);
+int dxmc_hip_host_free(void* p);
 
 /* ------------------------------------------------------------------ world */
 /* Voxel world, reference World<T> (world.hpp:37-227): x fastest layout

@@ -63,6 +63,19 @@ def test_large_transfers(device, dev_tables, orc):
     ca, cm = orc.lut_lookup(0, mat, e)
     assert np.max(np.abs(ga - ca) / ca) <= 1e-12
     assert np.max(np.abs(gm - cm) / cm) <= 1e-12
+    # into page-locked memory from dxmc_hip_host_alloc the copy is direct
+    import ctypes as C
+    lib = device.lib
+    p = C.c_void_p()
+    assert lib.dxmc_hip_host_alloc(n * 4 * 4 + 4096, C.byref(p)) == 0
+    try:
+        out = np.ctypeslib.as_array(C.cast(C.c_void_p(p.value + 4096), C.POINTER(C.c_uint32)), shape=(n * 4,))
+        out[:] = 0
+        device._check(lib.dxmc_hip_rng_streams(device.ctx, 42, 12345, n, 4, out.ctypes.data_as(C.POINTER(C.c_uint32))))
+        assert np.array_equal(out.reshape(n, 4), oracle.rng_streams(42, 12345, n, 4))
+    finally:
+        assert lib.dxmc_hip_host_free(p) == 0
+    assert lib.dxmc_hip_host_free(C.c_void_p(12345)) != 0
 
 
 def _energies(n, rng):
