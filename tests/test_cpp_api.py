@@ -6,6 +6,11 @@ linked to libdxmc_host.so / libdxmc_hip.so.
 CPU: without a GPU the program fails loudly (no CPU fallback).
 GPU: its FP64 result equals the oracle's on identical streams (step and event
 counts, energy per slab to 1e-9); its FP32 result agrees to 1e-3.
+
+tests/cpp/progress_preview.cpp watches a ProgressBar from a second thread
+while Transport runs (reference validation/validation.cpp:270-285): the
+exposure counter only grows and ends at the total, and dose previews from
+the device arrive during the run with the projected plane's size.
 """
 import ctypes as C
 import json
@@ -18,6 +23,7 @@ import pytest
 from dxmclib_amd import backend, build, capi
 
 EXE = os.path.join(build.PROGRAM_DIR, "layered_pencil")
+PREVIEW_EXE = os.path.join(build.PROGRAM_DIR, "progress_preview")
 NAMES = ["Air, Dry (near sea level)", "Water, Liquid", "Al"]
 
 
@@ -81,3 +87,31 @@ def test_program_matches_oracle():
     print(f"fp32 {f}", flush=True)
     for name in ("water", "aluminium"):
         assert abs(f[name] - g[name]) <= 1e-3 * g[name]
+
+
+def test_preview_program_fails_loudly_without_gpu():
+    import torch
+    if torch.cuda.is_available():
+        pytest.skip("a GPU is present")
+    assert os.path.exists(PREVIEW_EXE), "build() first"
+    r = subprocess.run([PREVIEW_EXE, "2"], capture_output=True, text=True, timeout=300)
+    assert r.returncode == 1
+    assert "HIP" in r.stderr and r.stdout == ""
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("axis", [0, 1, 2])
+def test_progress_bar_preview(axis):
+    assert os.path.exists(PREVIEW_EXE), "build() first"
+    r = subprocess.run([PREVIEW_EXE, str(axis)], capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0, r.stderr
+    g = json.loads(r.stdout)
+    print(g, flush=True)
+    dims = (40, 36, 32)  # x, y, z of the program's world
+    plane = [d for k, d in enumerate(dims) if k != axis]
+    assert g["histories"] == 50 * 20000
+    assert g["exposures"] == g["total_exposures"] == 50
+    assert g["monotone"]
+    assert g["mid_run"] >= 1
+    assert [g["width"], g["height"]] == plane
+    assert g["max_pixel"] == 255
