@@ -331,6 +331,7 @@ HOST_SYMBOLS = {
     "dxmc_host_transport": (C.c_int, [C.POINTER(HostWorld), C.POINTER(SourceSpec), C.POINTER(HostTransportCfg), f64p, u32p, f64p,
                                       C.POINTER(RunStats), f64p, C.POINTER(C.c_uint64)]),
     "dxmc_host_material_data": (C.c_int, [C.c_char_p, C.c_double, f64p, f64p]),
+    "dxmc_host_material_scatter_factors": (C.c_int, [C.c_char_p, C.c_uint64, f64p, f64p, f64p]),
     "dxmc_host_ctdi_phantom": (C.c_int, [C.c_uint64, C.POINTER(C.c_uint64), f64p, C.POINTER(C.c_uint8), f64p, C.POINTER(C.c_uint8),
                                          C.POINTER(C.c_uint8), C.c_uint64]),
     "dxmc_host_server_create": (C.c_int, [C.POINTER(ServerCfg), C.POINTER(C.c_void_p)]),

@@ -453,4 +453,20 @@ int dxmc_host_material_data(const char* name, double E, double out[5], double* d
         *density = m.standardDensity();
     return DXMC_OK;
 }
+
+int dxmc_host_material_scatter_factors(const char* name, std::uint64_t n, const double* q, double* ff2, double* sf)
+{
+    if (!name || (n && !q))
+        return DXMC_ERR_ARGUMENT;
+    Material m(name);
+    if (!m.isValid())
+        return fail(DXMC_ERR_ARGUMENT, std::string("material_scatter_factors: unknown material ") + name);
+    for (std::uint64_t i = 0; i < n; ++i) {
+        if (ff2)
+            ff2[i] = m.getRayleightFormFactorSquared(q[i]);
+        if (sf)
+            sf[i] = m.getComptonNormalizedScatterFactor(q[i]);
+    }
+    return DXMC_OK;
+}
 }

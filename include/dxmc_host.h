@@ -130,6 +130,13 @@ int dxmc_host_transport(const dxmc_host_world* world, const dxmc_source_spec* so
  * absorption [cm2/g] at energy E (keV) and the standard density. */
 int dxmc_host_material_data(const char* name, double energy, double out[5], double* density);
 
+/* Scatter factor probes at n momentum transfers q (1/Angstrom, sin(theta/2)
+ * over wavelength): the squared Rayleigh form factor and the normalised
+ * Compton scatter function the Livermore samplers are built from
+ * (Material::getRayleightFormFactorSquared, getComptonNormalizedScatterFactor,
+ * reference material.hpp:72-73). Either output may be NULL. */
+int dxmc_host_material_scatter_factors(const char* name, uint64_t n, const double* q, double* form_factor_squared, double* scatter_function);
+
 /* The CTDI phantom the CT sources calibrate on (CTDIPhantom, reference
  * world.hpp:229-390): PMMA cylinder of 'diameter_mm' with five air holes,
  * 1 x 1 x 2.5 mm voxels, materials 0 air, 1 air (holes), 2 PMMA. hole[i] is

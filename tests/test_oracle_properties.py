@@ -8,6 +8,9 @@ Covered here (reference file:line in each test's docstring):
   attenuation LUT exact at nodes / log-log linear / lagged extrapolation (Q2)
   Woodcock majorant >= rho_max * mu and majorant independence of results
   Klein-Nishina and Thomson angular distributions (chi-square)
+  Livermore (L=1) Rayleigh and Compton distributions against the squared form
+  factor and the scatter function probed from the material (chi-square), and
+  the impulse-model (L=2) fluorescence lines and their emission angle
   energy conservation of every interaction kind and model
   forced vs naive interaction expectation (testtransport.cpp:53-114)
   uncollided attenuation along a pencil beam, exp(-mu z)
@@ -364,3 +367,117 @@ def test_normalisation_closed_forms():
         ed = np.where(dens > 0, d * 3.0 / mass, 0)
     assert np.allclose(dn, ed)
     assert np.all(dn[:5] == 0) and np.all(vn[:5] == 0)
+
+
+HC_KEV_ANGSTROM = 12.398419843  # CODATA 2018 h*c in keV*Angstrom
+
+
+def _scatter_factors(name, q):
+    from dxmclib_amd import capi
+    q = np.ascontiguousarray(q, dtype=np.float64)
+    ff2, sf = np.zeros(len(q)), np.zeros(len(q))
+    st = capi.host().dxmc_host_material_scatter_factors(name.encode(), len(q), _p(q, C.c_double), _p(ff2, C.c_double), _p(sf, C.c_double))
+    assert st == 0
+    return ff2, sf
+
+
+def _chi2(samples, edges, x, pdf):
+    """Chi-square of a histogram of samples against a density tabulated at x."""
+    cdf = np.concatenate([[0], np.cumsum((pdf[1:] + pdf[:-1]) / 2 * np.diff(x))])
+    cdf /= cdf[-1]
+    expected = np.diff(np.interp(edges, x, cdf)) * len(samples)
+    observed, _ = np.histogram(samples, edges)
+    keep = expected > 5
+    return np.sum((observed[keep] - expected[keep]) ** 2 / expected[keep])
+
+
+def test_livermore_rayleigh_distribution(orc):
+    """transport.hpp:280-297 with L=1: q^2 drawn from the squared form factor
+    (attenuationlut.hpp:212-237), cos = 1 - 2 (q hc / E)^2 with the momentum
+    transfer q = sin(theta/2) E / hc, kept with probability (1 + cos^2)/2.
+    Since q^2 is linear in cos, the kept cosines have density
+    (1 + cos^2) F^2(q(cos)), F^2 probed from the material directly (not from
+    the sampling table). The two plausible mistakes fail: Thomson without the
+    form factor, and F instead of F^2."""
+    n = 200000
+    E = 30.0
+    p = _particles(n, E, 11)
+    c = _cos(p, orc.interact(1, 2, np.ones(n), p, seed=12)[0])
+    x = np.linspace(-1, 1, 40001)
+    q = E / HC_KEV_ANGSTROM * np.sqrt((1 - x) / 2)
+    ff2, _ = _scatter_factors(NAMES[1], q)
+    # the sampling table ends where F^2 has fallen below 0.001 (negligible mass)
+    edges = np.linspace(-1, 1, 41)
+    chi2 = _chi2(c, edges, x, (1 + x * x) * ff2)
+    assert chi2 < 80, chi2  # <= 40 bins
+    assert _chi2(c, edges, x, 1 + x * x) > 1000
+    assert _chi2(c, edges, x, (1 + x * x) * np.sqrt(ff2)) > 1000
+
+
+def test_livermore_compton_distribution(orc):
+    """transport.hpp:310-338 with L=1: Klein-Nishina kept with the
+    normalised scatter function S(q) at q = E/hc sqrt((1 - cos)/2). The
+    reduced energy eps = E'/E is proposed uniformly on [1/(1+2k), 1], so the
+    kept eps has density (1/eps + eps - sin^2) S(q(eps)), S probed from the
+    material directly. At 20 keV in bone S suppresses forward scattering
+    strongly: Klein-Nishina alone, or S at the wrong angle (sin theta for
+    sin theta/2), fails."""
+    n = 200000
+    E = 20.0
+    p = _particles(n, E, 13)
+    out, _, _ = orc.interact(1, 1, np.ones(n), p, seed=14)
+    eps = out[:, 6] / E
+    k = E / 510.9989461
+    emin = 1 / (1 + 2 * k)
+    x = np.linspace(emin, 1, 40001)
+    t = (1 - x) / (k * x)
+    cos = 1 - t
+    kn = 1 / x + x - t * (2 - t)
+    _, sf = _scatter_factors(NAMES[1], E / HC_KEV_ANGSTROM * np.sqrt((1 - cos) / 2))
+    edges = np.linspace(emin, 1, 41)
+    chi2 = _chi2(eps, edges, x, kn * sf)
+    assert chi2 < 80, chi2
+    assert _chi2(eps, edges, x, kn) > 1000
+    _, sf_wrong = _scatter_factors(NAMES[1], E / HC_KEV_ANGSTROM * np.sqrt(np.clip(1 - cos * cos, 0, None)))
+    assert _chi2(eps, edges, x, kn * sf_wrong) > 1000
+
+
+def test_fluorescence_lines(orc):
+    """transport.hpp:223-262 with L=2: the shell is chosen among those bound
+    below E in proportion to their photoionisation probability; it emits
+    with its fluorescence yield one of its three lines (the third takes the
+    remaining probability), in a direction whose polar angle is uniform on
+    [0, pi]. Expected probabilities of each outgoing energy follow from the
+    shell table in closed form."""
+    n = 400000
+    E = 60.0
+    m = 1  # bone: calcium and phosphorus K shells
+    sh = orc.arrays()["shells"][m * 12:(m + 1) * 12]
+    binding, prob, yld, lp, le = sh[:, 0], sh[:, 3], sh[:, 4], sh[:, 5:8], sh[:, 8:11]
+    w = np.where(E > binding, prob, 0.0)
+    w = w / w.sum()
+    line_p = np.stack([lp[:, 0], lp[:, 1], np.clip(1 - lp[:, 0] - lp[:, 1], 0, None)], axis=1)
+    emits = (binding >= 1.0) & (binding < E)
+    expected = {}
+    for s in np.nonzero(emits & (w > 0))[0]:
+        for j in range(3):
+            pr = w[s] * yld[s] * line_p[s, j]
+            if pr > 0:
+                expected[le[s, j]] = expected.get(le[s, j], 0.0) + pr
+    assert expected, "bone has shells that fluoresce at 60 keV"
+    p = _particles(n, E, 15)
+    out, e, _ = orc.interact(2, 0, np.full(n, m), p, seed=16)
+    fl = out[:, 6] > 0
+    energies = out[fl, 6]
+    for energy, pr in expected.items():
+        observed = np.count_nonzero(energies == energy)
+        sigma = math.sqrt(n * pr * (1 - pr)) + 1
+        assert abs(observed - n * pr) < 5 * sigma, (energy, observed, n * pr)
+    assert abs(fl.sum() - n * sum(expected.values())) < 5 * math.sqrt(n * sum(expected.values())) + 1
+    assert np.all(np.isin(energies, list(expected)))
+    # the polar angle of emission is uniform on [0, pi]
+    theta = np.arccos(np.clip(_cos(p[fl], out[fl]), -1, 1))
+    edges = np.linspace(0, np.pi, 21)
+    obs, _ = np.histogram(theta, edges)
+    exp_ = len(theta) / 20
+    assert np.sum((obs - exp_) ** 2 / exp_) < 50
