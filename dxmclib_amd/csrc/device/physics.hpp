@@ -150,10 +150,21 @@ struct World {
     uint32_t codeBytes, pad;
 };
 
+// One voxel's accumulators in 32 aligned bytes: the three atomics of a
+// scoring event land in one cache line (separate arrays cost three random
+// lines per event). Normalisation and download write the per-voxel arrays
+// of Result from these.
+struct alignas(32) VoxelScore {
+    double dose; // energy imparted, eV x weight
+    double variance; // sum of squares of the same
+    uint32_t events;
+    uint32_t pad0;
+    uint64_t pad1;
+};
+static_assert(sizeof(VoxelScore) == 32, "one voxel score is 32 bytes");
+
 struct Tally {
-    double* dose;
-    uint32_t* nEvents;
-    double* variance;
+    VoxelScore* voxels;
 };
 
 // tally plus this thread's event counter
@@ -493,9 +504,10 @@ __device__ inline uint64_t voxelIndex(const World<T>& w, const T* pos)
 // safeValueAdd x3 (transport.hpp:208-214): hardware FP64 / u32 atomics
 __device__ inline void score(Scorer& sc, uint64_t idx, double e)
 {
-    unsafeAtomicAdd(sc.tally.dose + idx, e);
-    atomicAdd(sc.tally.nEvents + idx, 1u);
-    unsafeAtomicAdd(sc.tally.variance + idx, e * e);
+    VoxelScore* v = sc.tally.voxels + idx;
+    unsafeAtomicAdd(&v->dose, e);
+    atomicAdd(&v->events, 1u);
+    unsafeAtomicAdd(&v->variance, e * e);
     ++sc.events;
 }
 

@@ -141,24 +141,31 @@ __global__ void sampleParticles(World<T> w, Tables<T> t, Exposure<T> x, uint64_t
 }
 
 // normalizeScoring (transport.hpp:780-794) / energyImpartedToDose (:796-816)
-__global__ void normalizeTally(Tally tally, uint64_t n, int mode, uint64_t nHist, double voxelVolume, double calibration, const double* dens64)
+// from the voxel scores into the result arrays: mode < 0 copies the sums
+__global__ void normalizeTally(const VoxelScore* __restrict__ scores, double* __restrict__ dose, uint32_t* __restrict__ events,
+    double* __restrict__ variance, uint64_t n, int mode, uint64_t nHist, double voxelVolume, double calibration, const double* dens64)
 {
     const uint64_t i = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
     if (i >= n)
         return;
-    if (mode == DXMC_OUTPUT_EV_PER_HISTORY) {
+    const VoxelScore v = scores[i];
+    events[i] = v.events;
+    if (mode < 0) {
+        dose[i] = v.dose;
+        variance[i] = v.variance;
+    } else if (mode == DXMC_OUTPUT_EV_PER_HISTORY) {
         const double hInv = 1.0 / static_cast<double>(nHist - 1);
         const double hd = 1e3 / static_cast<double>(nHist);
         const double hv = 1e6 / static_cast<double>(nHist);
-        const double d = tally.dose[i] * hd;
-        tally.dose[i] = d;
-        tally.variance[i] = (tally.variance[i] * hv - d * d) * hInv;
+        const double d = v.dose * hd;
+        dose[i] = d;
+        variance[i] = (v.variance * hv - d * d) * hInv;
     } else {
         const double de = dens64[i];
         const double mass = de * voxelVolume * 0.001;
         const double factor = calibration / mass;
-        tally.dose[i] = de > 0.0 ? tally.dose[i] * factor : 0.0;
-        tally.variance[i] = de > 0.0 ? tally.variance[i] * factor * factor : 0.0;
+        dose[i] = de > 0.0 ? v.dose * factor : 0.0;
+        variance[i] = de > 0.0 ? v.variance * factor * factor : 0.0;
     }
 }
 
@@ -166,7 +173,10 @@ __global__ void normalizeTally(Tally tally, uint64_t n, int mode, uint64_t nHist
 // (x fastest in the grid). Along y or z one thread per pixel walks the
 // axis, reading consecutive x across the wave; along x one wave per pixel
 // reduces the row.
-__global__ __launch_bounds__(256) void tallyMip(const double* __restrict__ dose, uint32_t nx, uint32_t ny, uint32_t nz, int axis, float* __restrict__ image)
+// dose[i * stride] is voxel i's dose (stride 4 over the voxel scores, 1
+// over the result array)
+__global__ __launch_bounds__(256) void tallyMip(const double* __restrict__ dose, uint32_t stride, uint32_t nx, uint32_t ny, uint32_t nz, int axis,
+    float* __restrict__ image)
 {
     const uint64_t slice = uint64_t(nx) * ny;
     if (axis == 0) {
@@ -174,10 +184,10 @@ __global__ __launch_bounds__(256) void tallyMip(const double* __restrict__ dose,
         const uint32_t lane = threadIdx.x % 64;
         if (pixel >= uint64_t(ny) * nz)
             return;
-        const double* row = dose + pixel * nx; // pixel = y + ny * z
+        const double* row = dose + pixel * nx * stride; // pixel = y + ny * z
         double m = 0;
         for (uint32_t x = lane; x < nx; x += 64)
-            m = fmax(m, row[x]);
+            m = fmax(m, row[uint64_t(x) * stride]);
         for (int off = 32; off > 0; off >>= 1)
             m = fmax(m, __shfl_xor(m, off));
         if (lane == 0)
@@ -189,30 +199,27 @@ __global__ __launch_bounds__(256) void tallyMip(const double* __restrict__ dose,
         if (pixel >= uint64_t(nx) * nz)
             return;
         const uint64_t x = pixel % nx, z = pixel / nx;
-        const double* p = dose + z * slice + x;
+        const double* p = dose + (z * slice + x) * stride;
         double m = 0;
         for (uint32_t y = 0; y < ny; ++y)
-            m = fmax(m, p[uint64_t(y) * nx]);
+            m = fmax(m, p[uint64_t(y) * nx * stride]);
         image[pixel] = static_cast<float>(m);
         return;
     }
     if (pixel >= slice) // axis 2, pixel = x + nx * y
         return;
-    const double* p = dose + pixel;
+    const double* p = dose + pixel * stride;
     double m = 0;
     for (uint32_t z = 0; z < nz; ++z)
-        m = fmax(m, p[uint64_t(z) * slice]);
+        m = fmax(m, p[uint64_t(z) * slice * stride]);
     image[pixel] = static_cast<float>(m);
 }
 
-__global__ void fillZero(double* a, uint32_t* b, double* c, uint64_t n)
+__global__ void fillZero(VoxelScore* scores, uint64_t n)
 {
     const uint64_t i = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (i < n) {
-        a[i] = 0;
-        b[i] = 0;
-        c[i] = 0;
-    }
+    if (i < n)
+        scores[i] = VoxelScore {};
 }
 
 uint32_t gridFor(uint64_t n, uint32_t block) { return static_cast<uint32_t>((n + block - 1) / block); }
@@ -433,9 +440,14 @@ int dxmc_hip_tally_create(dxmc_ctx* c, uint64_t n, dxmc_tally** out)
     t->ctx = c;
     t->n = n;
     int st;
-    if ((st = c->alloc(&t->dose, n * sizeof(double))) || (st = c->alloc(&t->nEvents, n * sizeof(uint32_t))) || (st = c->alloc(&t->variance, n * sizeof(double)))) {
+    if ((st = c->alloc(&t->scores, n * sizeof(VoxelScore))) || (st = c->alloc(&t->dose, n * sizeof(double)))
+        || (st = c->alloc(&t->nEvents, n * sizeof(uint32_t))) || (st = c->alloc(&t->variance, n * sizeof(double)))) {
         dxmc_hip_tally_destroy(c, t);
         return st;
+    }
+    if (reinterpret_cast<uintptr_t>(t->scores) % alignof(VoxelScore) != 0) {
+        dxmc_hip_tally_destroy(c, t);
+        return c->fail(DXMC_ERR_MEMORY, "tally_create: voxel scores not 32-byte aligned");
     }
     if ((st = dxmc_hip_tally_reset(c, t))) {
         dxmc_hip_tally_destroy(c, t);
@@ -453,6 +465,7 @@ int dxmc_hip_tally_destroy(dxmc_ctx* c, dxmc_tally* t)
         return c->fail(DXMC_ERR_STATE, "tally_destroy: tally belongs to another context");
     hipSetDevice(c->device);
     hipStreamSynchronize(c->stream);
+    c->release(t->scores);
     c->release(t->dose);
     c->release(t->nEvents);
     c->release(t->variance);
@@ -466,9 +479,9 @@ int dxmc_hip_tally_reset(dxmc_ctx* c, dxmc_tally* t)
     if (!c || !t || t->ctx != c)
         return DXMC_ERR_ARGUMENT;
     DXMC_CHECK(c, hipSetDevice(c->device));
-    hipLaunchKernelGGL(fillZero, dim3(gridFor(t->n, 256)), dim3(256), 0, c->stream, static_cast<double*>(t->dose), static_cast<uint32_t*>(t->nEvents),
-        static_cast<double*>(t->variance), t->n);
+    hipLaunchKernelGGL(fillZero, dim3(gridFor(t->n, 256)), dim3(256), 0, c->stream, static_cast<VoxelScore*>(t->scores), t->n);
     DXMC_CHECK(c, hipGetLastError());
+    t->arraysCurrent = false;
     return DXMC_OK;
 }
 
@@ -477,6 +490,10 @@ int dxmc_hip_tally_download(dxmc_ctx* c, const dxmc_tally* t, double* dose, uint
     if (!c || !t || t->ctx != c)
         return DXMC_ERR_ARGUMENT;
     DXMC_CHECK(c, hipSetDevice(c->device));
+    if (!t->arraysCurrent) { // the sums as they stand
+        if (int st = launchNormalize(c, t, nullptr, -1, 0, 0.0))
+            return st;
+    }
     DXMC_CHECK(c, hipStreamSynchronize(c->stream));
     if (dose)
         DXMC_CHECK(c, hipMemcpyAsync(dose, t->dose, t->n * sizeof(double), hipMemcpyDeviceToHost, c->stream));
@@ -518,12 +535,7 @@ int dxmc_hip_tally_normalize(dxmc_ctx* c, dxmc_tally* t, const dxmc_world* w, in
     if (mode == DXMC_OUTPUT_EV_PER_HISTORY && nHist < 2)
         return c->fail(DXMC_ERR_ARGUMENT, "tally_normalize: need at least two histories");
     DXMC_CHECK(c, hipSetDevice(c->device));
-    const double vv = w ? w->spacing[0] * w->spacing[1] * w->spacing[2] / 1000.0 : 0.0;
-    Tally tl { static_cast<double*>(t->dose), static_cast<uint32_t*>(t->nEvents), static_cast<double*>(t->variance) };
-    hipLaunchKernelGGL(normalizeTally, dim3(gridFor(t->n, 256)), dim3(256), 0, c->stream, tl, t->n, mode, nHist, vv, calibration,
-        w ? static_cast<const double*>(w->density) : nullptr);
-    DXMC_CHECK(c, hipGetLastError());
-    return DXMC_OK;
+    return launchNormalize(c, t, w, mode, nHist, calibration);
 }
 
 // -------------------------------------------------------------- transport
@@ -656,12 +668,12 @@ int dxmc_hip_interact(dxmc_ctx* c, const dxmc_tables* t, int32_t precision, int3
         if (mat[i] >= t->nMaterials || !(particles[i * 8 + 6] > 0))
             return c->fail(DXMC_ERR_ARGUMENT, "interact: material out of range or energy not positive");
     DXMC_CHECK(c, hipSetDevice(c->device));
-    DeviceBuffer dm(c), dp(c), de(c), da(c), td(c), tn(c), tv(c);
+    DeviceBuffer dm(c), dp(c), de(c), da(c), ts(c);
     int st;
-    if ((st = dm.upload(mat, n * 4)) || (st = dp.upload(particles, n * 64)) || (st = de.alloc(n * 8)) || (st = da.alloc(n * 4)) || (st = td.zeros(n * 8))
-        || (st = tn.zeros(n * 4)) || (st = tv.zeros(n * 8)))
+    if ((st = dm.upload(mat, n * 4)) || (st = dp.upload(particles, n * 64)) || (st = de.alloc(n * 8)) || (st = da.alloc(n * 4))
+        || (st = ts.zeros(n * sizeof(VoxelScore))))
         return st;
-    Tally tl { static_cast<double*>(td.p), static_cast<uint32_t*>(tn.p), static_cast<double*>(tv.p) };
+    Tally tl { static_cast<VoxelScore*>(ts.p) };
     if (precision == DXMC_PRECISION_FP64)
         launchInteract<double>(L, t->f64, kind, n, static_cast<const uint32_t*>(dm.p), fp, seed, first, static_cast<double*>(dp.p),
             static_cast<double*>(de.p), static_cast<int32_t*>(da.p), tl, c->stream);
@@ -675,7 +687,14 @@ int dxmc_hip_interact(dxmc_ctx* c, const dxmc_tables* t, int32_t precision, int3
     if (int e = c->download(aliveOut, da.p, n * 4))
         return e;
     // kinds 0-2 return the energy deposited, 3-4 the tally the dispatch scored into
-    return c->download(energyOut, kind <= 2 ? de.p : td.p, n * 8);
+    if (kind <= 2)
+        return c->download(energyOut, de.p, n * 8);
+    std::vector<VoxelScore> scores(n);
+    if (int e = c->download(scores.data(), ts.p, n * sizeof(VoxelScore)))
+        return e;
+    for (uint64_t i = 0; i < n; ++i)
+        energyOut[i] = scores[i].dose;
+    return DXMC_OK;
 }
 
 int dxmc_hip_sample_particles(dxmc_ctx* c, const dxmc_world* w, const dxmc_tables* t, int32_t precision, const dxmc_exposure* x, uint64_t seed,
@@ -737,9 +756,24 @@ int launchTallyMip(dxmc_ctx* c, const dxmc_tally* tally, const dxmc_world* w, in
     width = axis == 0 ? ny : nx;
     height = axis == 2 ? ny : nz;
     const uint64_t threads = axis == 0 ? width * height * 64 : width * height;
-    hipLaunchKernelGGL(tallyMip, dim3(gridFor(threads, 256)), dim3(256), 0, c->stream, static_cast<const double*>(tally->dose), nx, ny, nz, axis,
-        image);
+    // normalised arrays when the tally was normalised since its last run,
+    // else the dose sums
+    const bool arrays = tally->arraysCurrent;
+    const double* dose = arrays ? static_cast<const double*>(tally->dose) : &static_cast<const VoxelScore*>(tally->scores)->dose;
+    hipLaunchKernelGGL(tallyMip, dim3(gridFor(threads, 256)), dim3(256), 0, c->stream, dose, arrays ? 1u : 4u, nx, ny, nz, axis, image);
     return c->check(hipGetLastError());
+}
+
+int launchNormalize(dxmc_ctx* c, const dxmc_tally* t, const dxmc_world* w, int mode, uint64_t nHist, double calibration)
+{
+    const double vv = w ? w->spacing[0] * w->spacing[1] * w->spacing[2] / 1000.0 : 0.0;
+    hipLaunchKernelGGL(normalizeTally, dim3(gridFor(t->n, 256)), dim3(256), 0, c->stream, static_cast<const VoxelScore*>(t->scores),
+        static_cast<double*>(t->dose), static_cast<uint32_t*>(t->nEvents), static_cast<double*>(t->variance), t->n, mode, nHist, vv, calibration,
+        w ? static_cast<const double*>(w->density) : nullptr);
+    if (int e = c->check(hipGetLastError()))
+        return e;
+    t->arraysCurrent = true;
+    return DXMC_OK;
 }
 
 int runTransport(dxmc_ctx* c, const dxmc_world* w, const dxmc_tables* t, const dxmc_exposure* x, uint64_t nExp, const dxmc_run_cfg& cfg,
@@ -774,7 +808,8 @@ int runTransport(dxmc_ctx* c, const dxmc_world* w, const dxmc_tables* t, const d
         return st;
     const bool counters = (cfg.flags & 1u) != 0 || stats;
     Counters* ctr = counters ? static_cast<Counters*>(rb.counters.p) : nullptr;
-    const Tally tl { static_cast<double*>(tally->dose), static_cast<uint32_t*>(tally->nEvents), static_cast<double*>(tally->variance) };
+    const Tally tl { static_cast<VoxelScore*>(tally->scores) };
+    tally->arraysCurrent = false;
     const uint64_t total = rb.totalHistories;
     uint64_t chunk = cfg.histories_per_launch;
     if (chunk == 0)

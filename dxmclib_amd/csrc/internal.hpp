@@ -212,9 +212,15 @@ struct dxmc_tables {
 struct dxmc_tally {
     dxmc_ctx* ctx = nullptr;
     uint64_t n = 0;
+    // the transport accumulates into scores (VoxelScore[n]); normalisation
+    // and download fill the per-voxel arrays from them
+    void* scores = nullptr;
     void* dose = nullptr;
     void* nEvents = nullptr;
     void* variance = nullptr;
+    // the arrays hold the scores (raw or normalised) since the last run or
+    // reset
+    mutable bool arraysCurrent = false;
 };
 
 namespace dxmc_backend {
@@ -343,6 +349,9 @@ int runTransport(dxmc_ctx* c, const dxmc_world* w, const dxmc_tables* t, const d
 // maximum intensity projection of the tally's dose along axis into a device
 // image (launch only, on the context's stream); width and height returned
 int launchTallyMip(dxmc_ctx* c, const dxmc_tally* tally, const dxmc_world* w, int axis, float* image, uint64_t& width, uint64_t& height);
+// fills the tally's result arrays from its voxel scores: normalised for an
+// output mode, the sums as they stand for mode -1
+int launchNormalize(dxmc_ctx* c, const dxmc_tally* t, const dxmc_world* w, int mode, uint64_t nHist, double calibration);
 // FP32 transport launch (transport_fp32.hip)
 void launchFp32(dxmc_ctx* c, const dxmc_run_cfg& cfg, const dxmc_dev::World<float>& w, const dxmc_dev::Tables<float>& t,
     const dxmc_dev::Exposure<float>* exps, const uint64_t* expStart, uint32_t nExp, uint64_t hb, uint64_t he, const dxmc_dev::Tally& tally,

@@ -71,12 +71,14 @@ struct Request {
 
 // device memory a request holds while admitted: FP64 world (u8 material,
 // f64 density, optional u8 measurement), FP32 voxel records (at most 8 B per
-// voxel plus a palette), tally (f64 dose + variance, u32 count), tables,
-// exposure buffers and the upload's temporaries (12 MB of palette hash)
+// voxel plus a palette), bounded-density codes (at most 2 B per voxel plus
+// their intervals), tally (32-byte voxel scores and the f64 dose, variance
+// and u32 count result arrays), tables, exposure buffers and the upload's
+// temporaries (12 MB of palette hash)
 uint64_t deviceBytes(const Request& r)
 {
     const uint64_t n = r.world.dims[0] * r.world.dims[1] * r.world.dims[2];
-    return n * (1 + 8 + 8 + (r.world.measurement ? 1 : 0)) + 65536 * 8 + n * 20 + (20u << 20) + r.prep.exposures.size() * 512;
+    return n * (1 + 8 + 8 + 2 + (r.world.measurement ? 1 : 0)) + 65536 * (8 + 16) + n * 52 + (20u << 20) + r.prep.exposures.size() * 512;
 }
 
 }

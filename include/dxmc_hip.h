@@ -179,18 +179,21 @@ typedef struct {
 
 /* ------------------------------------------------------------------- tally */
 /* Result<T> (transport.hpp:42-65): dose (energy imparted) f64, event count
- * u32 and per-event variance sum f64 per voxel. */
+ * u32 and per-event variance sum f64 per voxel. Runs accumulate into the
+ * tally's sums; normalisation writes its result arrays from them. */
 int dxmc_hip_tally_create(dxmc_ctx* ctx, uint64_t n_voxels, dxmc_tally** out);
 int dxmc_hip_tally_destroy(dxmc_ctx* ctx, dxmc_tally* tally);
 int dxmc_hip_tally_reset(dxmc_ctx* ctx, dxmc_tally* tally);
-/* Copy to host; any pointer may be NULL to skip that array. */
+/* Copy to host; any pointer may be NULL to skip that array. After a
+ * normalisation (and no run since) the normalised arrays, else the sums. */
 int dxmc_hip_tally_download(dxmc_ctx* ctx, const dxmc_tally* tally, double* dose, uint32_t* n_events, double* variance);
 /* Maximum intensity projection of a tally's dose array along axis (0 x,
  * 1 y, 2 z) of the world's grid, laid out as the run preview of
  * dxmc_hip_run_monitored; image holds the product of the other two
  * dimensions. */
 int dxmc_hip_tally_mip(dxmc_ctx* ctx, const dxmc_tally* tally, const dxmc_world* world, int32_t axis, float* image);
-/* In-place FP64 normalisation on the device:
+/* FP64 normalisation on the device, from the sums into the result arrays
+ * (the sums stay, so normalising again gives the same arrays):
  * EV_PER_HISTORY: normalizeScoring (transport.hpp:780-794),
  * DOSE: energyImpartedToDose with a calibration factor (transport.hpp:796-816). */
 int dxmc_hip_tally_normalize(dxmc_ctx* ctx, dxmc_tally* tally, const dxmc_world* world, int output_mode, uint64_t n_histories,
