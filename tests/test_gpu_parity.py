@@ -615,3 +615,57 @@ def test_transport_forced_fp64(device):
     rel32 = abs(d32[flagged].sum() - d[flagged].sum()) / d[flagged].sum()
     print(f"forced FP32: flagged rel {rel32:.2e}", flush=True)
     assert rel32 < 1e-3
+
+
+def test_tally_sums_and_result_arrays(device):
+    """Runs accumulate into the tally's voxel sums; normalisation
+    (transport.hpp:780-816) writes the result arrays from them, so repeating
+    it, in either mode, starts from the same sums. A download after a run
+    and before normalising returns the sums; the projection of a tally at
+    rest follows its normalised dose; a further run adds to the sums."""
+    print("tally sums and result arrays", flush=True)
+    rng = np.random.default_rng(21)
+    shape = (24, 20, 16)  # z, y, x
+    mat = rng.integers(0, 3, shape).astype(np.uint8)
+    dens = np.asarray(MAXD, dtype=np.float64)[mat]
+    spacing = (2.0, 1.5, 1.0)
+    ext = backend.world_extent(shape[::-1], spacing)
+    w = device.upload_world(mat, dens, spacing, ext, len(NAMES))
+    spec = backend.source_spec(capi.SOURCE_DX, tube_voltage=120.0, al_mm=2.5, sdd=1000.0, field_size=(40.0, 40.0),
+                               histories_per_exposure=20_000, total_exposures=5)
+    t = backend.HostTables(NAMES, MAXD, 120.0, 1.0)
+    exps = t.exposures(spec)
+    tt = device.upload_tables(t.desc())
+    n = mat.size
+    tally = device.tally(n)
+    try:
+        stats = device.run(w, tt, exps, tally, capi.PRECISION_FP32, 1, seed=5)
+        raw = device.download(tally, n)
+        assert raw[1].sum() == stats.events and raw[0].sum() > 0
+        again = device.download(tally, n)
+        assert all(np.array_equal(a, b) for a, b in zip(raw, again))
+        h = stats.histories
+        device.normalize(tally, w, capi.OUTPUT_EV_PER_HISTORY, h)
+        e1 = device.download(tally, n)
+        device.normalize(tally, w, capi.OUTPUT_EV_PER_HISTORY, h)
+        e2 = device.download(tally, n)
+        assert all(np.array_equal(a, b) for a, b in zip(e1, e2))
+        assert np.array_equal(e1[1], raw[1])
+        d, v = oracle.normalize(raw[0], raw[2], capi.OUTPUT_EV_PER_HISTORY, h)
+        assert np.allclose(e1[0], d, rtol=1e-13, atol=0)
+        assert np.allclose(e1[2], v, rtol=1e-9, atol=1e-12 * np.abs(v).max())
+        device.normalize(tally, w, capi.OUTPUT_DOSE, h, calibration=2.5)
+        g = device.download(tally, n)
+        d, v = oracle.normalize(raw[0], raw[2], capi.OUTPUT_DOSE, h, density=dens.ravel(), spacing=np.asarray(spacing), calibration=2.5)
+        assert np.allclose(g[0], d, rtol=1e-13, atol=0)
+        assert np.allclose(g[2], v, rtol=1e-12, atol=0)
+        mip = device.tally_mip(tally, w, shape[::-1], 2)
+        assert np.array_equal(mip, g[0].reshape(shape).max(axis=0).astype(np.float32))
+        more = device.run(w, tt, exps, tally, capi.PRECISION_FP32, 1, seed=6)
+        r2 = device.download(tally, n)
+        assert r2[1].sum() == stats.events + more.events
+        assert np.all(r2[1] >= raw[1])
+    finally:
+        device.destroy_tally(tally)
+        device.destroy_tables(tt)
+        device.destroy_world(w)
