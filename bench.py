@@ -127,8 +127,25 @@ def cpu_baseline(wl, tables, exposures, ext, L, seconds):
     t0 = time.perf_counter()
     r = orc.transport(world, sample, L=L, threads=threads)
     dt = time.perf_counter() - t0
+    # the FP32 restatement on the same sample
+    t0 = time.perf_counter()
+    r32 = orc.transport(world, sample, L=L, f32=True, threads=threads)
+    dt32 = time.perf_counter() - t0
     return {"value": r["histories"] / dt, "unit": "histories/s", "cores": threads, "kind": "port",
-            "sample": f"{len(sample)} of {len(exposures)} exposures (every {stride}th) x {per} histories, FP64, L={L}, {dt:.1f} s"}
+            "sample": f"{len(sample)} of {len(exposures)} exposures (every {stride}th) x {per} histories, FP64, L={L}, {dt:.1f} s",
+            "fp32_value": r32["histories"] / dt32, "host_cpu": host_cpu(), "host_threads": os.cpu_count()}
+
+
+def host_cpu():
+    """Model name of the host CPU (first /proc/cpuinfo entry)."""
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return None
 
 
 def serving(n_requests, rate, slots, prep_threads):
