@@ -230,8 +230,8 @@ def spectrum_of(spec):
 def main():
     args = parse()
     if args.gpus != 1:
-        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (the transport shards by histories; "
-              "see DESIGN.md for the multi-GPU design)", file=sys.stderr)
+        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (on several GPUs the transport would shard "
+              "by histories, with one tally reduction at the end)", file=sys.stderr)
         sys.exit(2)
     from dxmclib_amd import backend, capi
 

@@ -2,7 +2,7 @@
 
 Bars (tolerances stated per test):
   RNG streams                      bit-exact
-  LUT / majorant (FP64)            <= 1e-12 relative (the north-star bar is 1e-6)
+  LUT / majorant (FP64)            <= 1e-12 relative (the required bar is 1e-6)
   LUT / majorant (FP32)            <= 2e-6 relative to the FP64 oracle
   single interactions (FP64)       identical up to math-library rounding:
                                    >= 99 % of samples within 1e-9 relative
@@ -246,7 +246,7 @@ def test_transport_water_pencil_fp32_vs_fp64(device):
     # identical streams: same histories, so counts agree closely
     assert abs(s32.steps - s64.steps) <= 1e-3 * s64.steps
     assert abs(s32.events - s64.events) <= 1e-3 * s64.events
-    # below 0.1 % in voxels above 10 % of the maximum
+    # the accuracy target: below 0.1 % in voxels above 10 % of the maximum
     assert err.max() < 1e-3
 
 
