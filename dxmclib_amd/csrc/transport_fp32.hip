@@ -420,8 +420,9 @@ uint32_t residentBlocksFp32(int L, uint32_t waves, uint32_t ldsPerBlock)
     return static_cast<uint32_t>(resident);
 }
 
-// staging level of a run: the requested one, or by default all tables when
-// three blocks of them fit a CU, else all but the attenuation coefficients
+// staging level of a run: the requested one (lowered until it fits), or by
+// default the attenuation coefficients too when they cost no resident block,
+// else the sampling tables when they cost at most one, else none
 uint32_t stageLevelFp32(const dxmc_run_cfg& cfg, const Tables<float>& t)
 {
     // a block gets at most 64 KiB of dynamic LDS
@@ -434,14 +435,19 @@ uint32_t stageLevelFp32(const dxmc_run_cfg& cfg, const Tables<float>& t)
             --level;
         return level;
     }
-    // the most tables that keep the occupancy the registers allow: a
-    // resident block is worth more than the attenuation coefficients in LDS
-    // (CT spiral 2.84e9 vs 2.50e9, DX 4.07e9 vs 3.32e9 histories/s with
-    // 5 and 3 blocks per CU)
+    // The sampling tables are worth a resident block (whole body, 8
+    // materials: 1.04e9 histories/s with them and 4 blocks per CU against
+    // 0.98e9 without and 5); the attenuation coefficients are not worth two
+    // (CT spiral 2.84e9 vs 2.50e9, DX 4.07e9 vs 3.32e9 with 5 and 3 blocks)
+    // and are staged only when they cost none.
     const uint32_t best = residentBlocksFp32(cfg.low_energy_correction, cfg.waves_per_simd, bytes(STAGE_NONE));
-    for (uint32_t level = STAGE_ALL; level > STAGE_NONE; --level)
-        if (fits(level) && residentBlocksFp32(cfg.low_energy_correction, cfg.waves_per_simd, static_cast<uint32_t>(bytes(level))) >= best)
-            return level;
+    auto resident = [&](uint32_t level) {
+        return residentBlocksFp32(cfg.low_energy_correction, cfg.waves_per_simd, static_cast<uint32_t>(bytes(level)));
+    };
+    if (fits(STAGE_ALL) && resident(STAGE_ALL) >= best)
+        return STAGE_ALL;
+    if (fits(STAGE_SMALL) && resident(STAGE_SMALL) + 1 >= best)
+        return STAGE_SMALL;
     return STAGE_NONE;
 }
 
