@@ -63,21 +63,27 @@ __device__ inline uint64_t voxelIndexFast(const World<float>& w, const float* po
 // into LDS at kernel start, one copy per block: their lookups then cost LDS
 // latency instead of L2 latency, also while the world gathers keep evicting
 // them from L1. Staging level: 1 none, 2 all but the attenuation
-// coefficients, 3 all. The total-attenuation bounds stay in global memory:
-// in LDS they cost a resident block per CU.
+// coefficients, 3 all. Of the interaction tables only those of the run's
+// low-energy model are staged (RITA for L >= 1, the scatter-function spline
+// for L = 1, the shells for L = 2). The total-attenuation bounds stay in
+// global memory: in LDS they cost a resident block per CU.
 enum : uint32_t { STAGE_NONE = 1, STAGE_SMALL = 2, STAGE_ALL = 3 };
 
 __host__ __device__ constexpr uint32_t ldsAlign(uint64_t bytes) { return static_cast<uint32_t>((bytes + 15) & ~uint64_t(15)); }
 
-// bytes of LDS a staging level needs (the same walk as stageTables)
-__host__ __device__ inline uint32_t ldsBytes(const Tables<float>& t, uint32_t level)
+// bytes of LDS a staging level needs for low-energy model L (the same walk
+// as stageTables)
+__host__ __device__ inline uint32_t ldsBytes(const Tables<float>& t, uint32_t level, int L)
 {
     if (level <= STAGE_NONE)
         return 0;
     uint32_t b = 0;
-    b += ldsAlign(uint64_t(t.nMaterials) * 4 * 56 * sizeof(float));
-    b += ldsAlign(uint64_t(t.nMaterials) * 63 * sizeof(float));
-    b += ldsAlign(uint64_t(t.nMaterials) * 12 * sizeof(Shell<float>));
+    if (L >= 1)
+        b += ldsAlign(uint64_t(t.nMaterials) * 4 * 56 * sizeof(float));
+    if (L == 1)
+        b += ldsAlign(uint64_t(t.nMaterials) * 63 * sizeof(float));
+    if (L == 2)
+        b += ldsAlign(uint64_t(t.nMaterials) * 12 * sizeof(Shell<float>));
     b += ldsAlign(uint64_t(t.lutNodes) * 2 * sizeof(float));
     b += ldsAlign(uint64_t(t.lutNodes) * sizeof(double));
     b += ldsAlign(uint64_t(t.lutNodes) * sizeof(float));
@@ -108,15 +114,18 @@ __device__ inline const T* stageArray(const T* src, uint64_t count, unsigned cha
 // copies the tables into LDS and points t at the copies (all threads of the
 // block call it; ends with a barrier). The level is a template parameter so
 // that the compiler sees which pointers address LDS.
-template <uint32_t level>
+template <uint32_t level, int L>
 __device__ inline void stageTables(Tables<float>& t, unsigned char* lds)
 {
     if constexpr (level <= STAGE_NONE)
         return;
     unsigned char* c = lds;
-    t.rita = stageArray(t.rita, uint64_t(t.nMaterials) * 4 * 56, c);
-    t.spline = stageArray(t.spline, uint64_t(t.nMaterials) * 63, c);
-    t.shells = stageArray(t.shells, uint64_t(t.nMaterials) * 12, c);
+    if constexpr (L >= 1)
+        t.rita = stageArray(t.rita, uint64_t(t.nMaterials) * 4 * 56, c);
+    if constexpr (L == 1)
+        t.spline = stageArray(t.spline, uint64_t(t.nMaterials) * 63, c);
+    if constexpr (L == 2)
+        t.shells = stageArray(t.shells, uint64_t(t.nMaterials) * 12, c);
     t.lutMajorant = stageArray(t.lutMajorant, uint64_t(t.lutNodes) * 2, c);
     t.lutBound = stageArray(t.lutBound, t.lutNodes, c);
     t.lutAnchor = stageArray(t.lutAnchor, t.lutNodes, c);
@@ -172,7 +181,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(W, 8))) voi
 {
     extern __shared__ __align__(16) unsigned char lds[];
     Tables<float> t = tg;
-    stageTables<S>(t, lds);
+    stageTables<S, L>(t, lds);
     Particle<float> p;
     Rng rng { 0, 1 };
     float majInv = 0;
@@ -196,7 +205,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(W, 8))) voi
     const uint32_t lane = __lane_id();
 
     // the wave's queue of new photons, already in the world (LDS, SoA)
-    float* q = reinterpret_cast<float*>(lds + ldsBytes(tg, S) + (threadIdx.x / 64) * QUEUE_BYTES);
+    float* q = reinterpret_cast<float*>(lds + ldsBytes(tg, S, L) + (threadIdx.x / 64) * QUEUE_BYTES);
     uint32_t queued = 0;
 
     for (;;) {
@@ -427,7 +436,7 @@ uint32_t stageLevelFp32(const dxmc_run_cfg& cfg, const Tables<float>& t)
 {
     // a block gets at most 64 KiB of dynamic LDS
     constexpr uint32_t ldsPerBlock = 64 * 1024;
-    auto bytes = [&](uint32_t level) { return uint64_t(ldsBytes(t, level)) + WAVES_PER_BLOCK * QUEUE_BYTES; };
+    auto bytes = [&](uint32_t level) { return uint64_t(ldsBytes(t, level, cfg.low_energy_correction)) + WAVES_PER_BLOCK * QUEUE_BYTES; };
     auto fits = [&](uint32_t level) { return bytes(level) <= ldsPerBlock; };
     if (cfg.lds_tables >= STAGE_NONE && cfg.lds_tables <= STAGE_ALL) {
         uint32_t level = cfg.lds_tables;
@@ -468,7 +477,7 @@ void launchFp32(dxmc_ctx* c, const dxmc_run_cfg& cfg, const World<float>& w, con
         tables.lutSumBound = nullptr;
     }
     const uint32_t stage = stageLevelFp32(cfg, tables);
-    const uint32_t lds = ldsBytes(tables, stage) + WAVES_PER_BLOCK * QUEUE_BYTES;
+    const uint32_t lds = ldsBytes(tables, stage, cfg.low_energy_correction) + WAVES_PER_BLOCK * QUEUE_BYTES;
     // default: as many blocks as stay resident on a CU (registers and LDS)
     const uint32_t perCU = cfg.blocks_per_cu ? cfg.blocks_per_cu : residentBlocksFp32(cfg.low_energy_correction, waves, lds);
     const uint32_t regen = cfg.regeneration_threshold ? std::min<uint32_t>(cfg.regeneration_threshold, 64u) : 4u;
