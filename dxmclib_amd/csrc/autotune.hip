@@ -106,7 +106,7 @@ int autotune(dxmc_ctx* c, const dxmc_world* w, const dxmc_tables* t, const dxmc_
     double bestRate = st ? -1 : tr.rate(cfg, st);
     if (!st) {
         if (cfg.precision == DXMC_PRECISION_FP32) {
-            st = searchField(tr, cfg, &dxmc_run_cfg::waves_per_simd, { 6 }, bestRate);
+            st = searchField(tr, cfg, &dxmc_run_cfg::waves_per_simd, { 5, 6 }, bestRate);
             if (!st)
                 st = searchField(tr, cfg, &dxmc_run_cfg::lds_tables, { 1, 2, 3 }, bestRate);
             const uint32_t resident = residentBlocksFp32(cfg.low_energy_correction, cfg.waves_per_simd, 0);

@@ -147,8 +147,9 @@ __device__ inline void stageTables(Tables<float>& t, unsigned char* lds)
 constexpr float LN2F = 0.693147180559945309f;
 
 // per-wave queue of sampled photons in LDS: position, direction, energy,
-// weight, PCG state and increment, LUT segment and offset, 1/majorant
-constexpr int QUEUE_FIELDS = 15;
+// weight, PCG state and increment (the LUT point and majorant follow from
+// the energy when a lane takes the photon)
+constexpr int QUEUE_FIELDS = 12;
 constexpr uint32_t QUEUE_CAP = 64;
 constexpr uint32_t QUEUE_BYTES = QUEUE_FIELDS * QUEUE_CAP * 4;
 constexpr uint32_t WAVES_PER_BLOCK = 4;
@@ -172,8 +173,9 @@ __device__ inline bool roulette(Particle<float>& p, Rng& rng)
 }
 
 // W: minimum resident waves per SIMD the register allocation must allow
-// (1: unconstrained, about 5 with this kernel; 6 trades a few spills for
-// occupancy). S: LDS staging level. The autotuner picks among the variants.
+// (1: unconstrained, 81 VGPRs and 5 waves at L = 0, 1, 91 at L = 2; 6: at
+// most 80 VGPRs, spill-free at L = 0, 1). S: LDS staging level. The
+// autotuner picks among the variants.
 template <int L, int W, uint32_t S>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(W, 8))) void transportPersistent(World<float> w, Tables<float> tg, const Exposure<float>* exps, const uint64_t* expStart,
     uint32_t nExp, uint64_t hBegin, uint64_t hEnd, uint64_t seed, uint64_t firstHistory, Tally tally, Counters* ctr, unsigned long long* next,
@@ -230,6 +232,9 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(W, 8))) voi
                     if (lane == 0)
                         base = atomicAdd(next, static_cast<unsigned long long>(poolSize));
                     base = __shfl(base, 0);
+                    // wave-uniform: the pool bounds live in scalar registers
+                    base = (static_cast<unsigned long long>(__builtin_amdgcn_readfirstlane(static_cast<uint32_t>(base >> 32))) << 32)
+                        | __builtin_amdgcn_readfirstlane(static_cast<uint32_t>(base));
                     if (base >= span) {
                         claimedAll = true;
                         break;
@@ -257,12 +262,9 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(W, 8))) voi
             const uint64_t entered = __ballot(sampled);
             if (sampled) {
                 const uint32_t slot = queued + rankBelow(entered);
-                const LutPoint<float> nl = lutPoint(t, np.energy);
-                const float nm = majorantInverseAt(t, nl);
                 const float v[QUEUE_FIELDS] = { np.pos[0], np.pos[1], np.pos[2], np.dir[0], np.dir[1], np.dir[2], np.energy, np.weight,
                     __uint_as_float(static_cast<uint32_t>(nr.s)), __uint_as_float(static_cast<uint32_t>(nr.s >> 32)),
-                    __uint_as_float(static_cast<uint32_t>(nr.inc)), __uint_as_float(static_cast<uint32_t>(nr.inc >> 32)),
-                    __uint_as_float(nl.k), nl.x, nm };
+                    __uint_as_float(static_cast<uint32_t>(nr.inc)), __uint_as_float(static_cast<uint32_t>(nr.inc >> 32)) };
                 for (int f = 0; f < QUEUE_FIELDS; ++f)
                     q[f * QUEUE_CAP + slot] = v[f];
             }
@@ -286,8 +288,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(W, 8))) voi
                     p.weight = v[7];
                     rng.s = static_cast<uint64_t>(__float_as_uint(v[9])) << 32 | __float_as_uint(v[8]);
                     rng.inc = static_cast<uint64_t>(__float_as_uint(v[11])) << 32 | __float_as_uint(v[10]);
-                    lp = { __float_as_uint(v[12]), v[13] };
-                    majInv = v[14];
+                    lp = lutPoint(t, p.energy);
+                    majInv = majorantInverseAt(t, lp);
                     update = false;
                     historySteps = 0;
                     phase = FLYING;
@@ -398,9 +400,14 @@ namespace {
 using KernelFn = void (*)(World<float>, Tables<float>, const Exposure<float>*, const uint64_t*, uint32_t, uint64_t, uint64_t, uint64_t, uint64_t,
     Tally, Counters*, unsigned long long*, uint32_t, uint32_t, uint32_t);
 
+// waves 0: the default, 6 waves per SIMD where that fits 80 VGPRs without
+// spills (L = 0, 1: CT spiral 3.55e9 vs 3.34e9, DX 4.13e9 vs 3.90e9
+// histories/s); L = 2 would spill and keeps 5
 template <int L, uint32_t S>
 KernelFn variant(uint32_t waves)
 {
+    if (waves == 0)
+        waves = L == 2 ? 5 : 6;
     return waves >= 6 ? transportPersistent<L, 6, S> : transportPersistent<L, 1, S>;
 }
 

@@ -212,7 +212,8 @@ typedef struct {
     uint32_t flags; /* DXMC_RUN_* bits */
     uint32_t interaction_threshold; /* FP32: pending interactions per wave before they run, 0: tuned */
     uint32_t queue_depth; /* FP32: histories a wave claims per refill of its pool, 0: tuned */
-    uint32_t waves_per_simd; /* FP32: kernel variant by register budget, 5 (0) or 6 resident waves per SIMD */
+    uint32_t waves_per_simd; /* FP32: kernel variant by register budget, 5 or 6 resident waves per SIMD;
+                              * 0: 6 for L = 0, 1 and 5 for L = 2 */
     uint32_t lds_tables; /* FP32: sampling tables in LDS, 1 none, 2 all but the attenuation coefficients, 3 all (0: tuned) */
 } dxmc_run_cfg;
 
