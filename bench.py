@@ -261,7 +261,9 @@ def main():
         tune_report = None
         if not args.no_autotune:
             t1 = time.perf_counter()
-            tuning, rep = dev.autotune(w, t, exposures, capi.PRECISION_FP32, L, trial_histories=1 << 25, base=tuning)
+            # each trial is one launch of the size the timed run launches
+            trial = int(min(max(wl.histories, 1 << 20), 1 << 27))
+            tuning, rep = dev.autotune(w, t, exposures, capi.PRECISION_FP32, L, trial_histories=trial, base=tuning)
             tuning.flags = 1
             tune_report = {"seconds": time.perf_counter() - t1, "trials": rep.trials, "default_rate": rep.default_rate,
                            "best_rate": rep.best_rate, "regeneration_threshold": tuning.regeneration_threshold,
