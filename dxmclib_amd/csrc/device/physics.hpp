@@ -86,6 +86,9 @@ struct Tables {
     double lutLinearEnergy;
     const double* lutGrid; // log10 E of the nodes, both precisions
     const double* lutBound; // FP32: lowest energy of each segment on the FP64 path
+    // FP32: the same rounded up to float; for a float E, E >= b and E < b
+    // decide exactly as against b itself
+    const float* lutBoundUp;
     const float* lutAnchor; // FP32: log2 E of the nodes
     float lutLinearEnergy2, lutLinearStepInv2; // FP32: linear index start and 1/step, log2 units
     const T* lutCoef; // [mat][node][type][b,a]
@@ -303,10 +306,9 @@ __device__ inline LutPoint<float> lutPoint(const Tables<float>& t, float E)
         k = upperBound(t.lutAnchor, t.lutNodes, le2);
         k = k != t.lutNodes ? k : t.lutNodes - 1;
     }
-    const double e = E;
-    while (k + 1 < t.lutNodes && e >= t.lutBound[k + 1])
+    while (k + 1 < t.lutNodes && E >= t.lutBoundUp[k + 1])
         ++k;
-    while (k > 0 && e < t.lutBound[k])
+    while (k > 0 && E < t.lutBoundUp[k])
         --k;
     return { k, le2 - t.lutAnchor[k] };
 }

@@ -229,6 +229,7 @@ namespace {
         if ((st = upload(c, t, cast<double>(d.lut_grid, d.lut_nodes), &out.lutGrid)))
             return st;
         out.lutBound = nullptr;
+        out.lutBoundUp = nullptr;
         out.lutAnchor = nullptr;
         if (base2) {
             // segment bounds from the FP64 copy (built first), float anchors
@@ -267,6 +268,17 @@ namespace {
             const auto anchors = cast<float>(d.lut_grid, d.lut_nodes, LOG2_10);
             const auto sums = totalAttenuationBounds(coef, anchors, bounds, d.n_materials, d.lut_nodes);
             if ((st = upload(c, t, sums, &out.lutSumBound)))
+                return st;
+            // segment bounds rounded up to float: a float energy compares
+            // with them (>=, <) exactly as with the FP64 values
+            std::vector<float> up(d.lut_nodes);
+            for (size_t k = 0; k < up.size(); ++k) {
+                float f = static_cast<float>(bounds[k]);
+                if (static_cast<double>(f) < bounds[k])
+                    f = std::nextafter(f, HUGE_VALF);
+                up[k] = f;
+            }
+            if ((st = upload(c, t, up, &out.lutBoundUp)))
                 return st;
         }
         std::vector<T> maj(size_t(d.lut_nodes) * 2);

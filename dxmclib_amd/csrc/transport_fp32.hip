@@ -85,7 +85,7 @@ __host__ __device__ inline uint32_t ldsBytes(const Tables<float>& t, uint32_t le
     if (L == 2)
         b += ldsAlign(uint64_t(t.nMaterials) * 12 * sizeof(Shell<float>));
     b += ldsAlign(uint64_t(t.lutNodes) * 2 * sizeof(float));
-    b += ldsAlign(uint64_t(t.lutNodes) * sizeof(double));
+    b += ldsAlign(uint64_t(t.lutNodes) * sizeof(float));
     b += ldsAlign(uint64_t(t.lutNodes) * sizeof(float));
     b += ldsAlign(uint64_t(t.nSpectra + 1) * sizeof(uint32_t));
     b += ldsAlign(uint64_t(t.nSpectra ? t.nSpectra : 1) * sizeof(uint32_t));
@@ -127,7 +127,7 @@ __device__ inline void stageTables(Tables<float>& t, unsigned char* lds)
     if constexpr (L == 2)
         t.shells = stageArray(t.shells, uint64_t(t.nMaterials) * 12, c);
     t.lutMajorant = stageArray(t.lutMajorant, uint64_t(t.lutNodes) * 2, c);
-    t.lutBound = stageArray(t.lutBound, t.lutNodes, c);
+    t.lutBoundUp = stageArray(t.lutBoundUp, t.lutNodes, c);
     t.lutAnchor = stageArray(t.lutAnchor, t.lutNodes, c);
     t.specOffset = stageArray(t.specOffset, t.nSpectra + 1, c);
     t.specThreshold = stageArray(t.specThreshold, t.nSpectra ? t.nSpectra : 1, c);
