@@ -669,3 +669,42 @@ def test_tally_sums_and_result_arrays(device):
         device.destroy_tally(tally)
         device.destroy_tables(tt)
         device.destroy_world(w)
+
+
+@pytest.mark.parametrize("L", [1, 2])
+def test_fp32_variants_follow_the_same_histories(device, L):
+    """Every FP32 kernel variant and scheduling field (register budget of 5
+    or 6 waves per SIMD, LDS staging level 1-3, blocks per CU, queue depth,
+    regeneration and interaction thresholds, launch size) runs the same
+    histories: equal step, event and per-voxel event counts, dose equal to
+    FP64 atomic-order rounding."""
+    print(f"fp32 variants, L={L}", flush=True)
+    wl = workloads.dx_thorax(histories_per_exposure=3000, exposures=40)
+    ext = backend.world_extent(wl.material.shape[::-1], wl.spacing)
+    t = backend.HostTables(wl.names, [0.001205, 1.0, 0.95, 1.92, 0.26], 120.0, 1.0)
+    exps = t.exposures(wl.spec)
+    w = device.upload_world(wl.material, wl.density, wl.spacing, ext, len(wl.names))
+    tt = device.upload_tables(t.desc())
+    variants = [dict(), dict(waves_per_simd=5), dict(waves_per_simd=6), dict(lds_tables=1), dict(lds_tables=3),
+                dict(blocks_per_cu=2, queue_depth=64), dict(regeneration_threshold=1, interaction_threshold=64),
+                dict(waves_per_simd=6, lds_tables=3, histories_per_launch=20_000)]
+    try:
+        out = []
+        for v in variants:
+            cfg = capi.RunCfg()
+            for k, val in v.items():
+                setattr(cfg, k, val)
+            cfg.flags = capi.RUN_COUNTERS
+            tally = device.tally(wl.voxels)
+            stats = device.run(w, tt, exps, tally, capi.PRECISION_FP32, L, seed=11, tuning=cfg)
+            d, ev, _ = device.download(tally, wl.voxels)
+            device.destroy_tally(tally)
+            out.append((v, stats.steps, stats.events, d, ev))
+        ref = out[0]
+        for v, steps, events, d, ev in out[1:]:
+            assert (steps, events) == (ref[1], ref[2]), v
+            assert np.array_equal(ev, ref[4]), v
+            assert np.allclose(d, ref[3], rtol=1e-10, atol=0), v
+    finally:
+        device.destroy_tables(tt)
+        device.destroy_world(w)
