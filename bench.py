@@ -190,7 +190,15 @@ def serving(n_requests, rate, slots, prep_threads):
     publish = np.array([x.finished - x.released for x in infos])
     hist = sum(x.histories for x in infos)
     span = max(x.finished for x in infos) - min(x.submitted for x in infos)
-    return {"requests": len(infos), "rate_per_s": rate, "slots": slots, "prep_threads": prep_threads,
+    # what the slowest 1 % of requests spend their time on
+    tail = lat >= np.percentile(lat, 99)
+    sides = [r["side"] for r in reqs]
+    tail_phases = {"requests": int(tail.sum()), "voxels_mean": float(np.mean([sides[i] ** 3 for i in np.nonzero(tail)[0]])),
+                   "histories_mean": float(np.mean([infos[i].histories for i in np.nonzero(tail)[0]])),
+                   "prepare_s": float(prep[tail].mean()), "admission_wait_s": float(wait[tail].mean()), "upload_s": float(upload[tail].mean()),
+                   "transport_s": float(transport[tail].mean()), "download_s": float(download[tail].mean()),
+                   "release_s": float(release[tail].mean()), "publish_s": float(publish[tail].mean())}
+    return {"requests": len(infos), "rate_per_s": rate, "slots": slots, "prep_threads": prep_threads, "p99_tail": tail_phases,
             "latency_p50_s": float(np.percentile(lat, 50)), "latency_p99_s": float(np.percentile(lat, 99)),
             "latency_mean_s": float(lat.mean()), "prepare_mean_s": float(prep.mean()), "admission_wait_mean_s": float(wait.mean()),
             "device_mean_s": float(run.mean()), "upload_mean_s": float(upload.mean()), "transport_mean_s": float(transport.mean()),

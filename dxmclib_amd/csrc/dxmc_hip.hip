@@ -494,13 +494,15 @@ int dxmc_hip_tally_download(dxmc_ctx* c, const dxmc_tally* t, double* dose, uint
         if (int st = launchNormalize(c, t, nullptr, -1, 0, 0.0))
             return st;
     }
-    DXMC_CHECK(c, hipStreamSynchronize(c->stream));
-    if (dose)
-        DXMC_CHECK(c, hipMemcpyAsync(dose, t->dose, t->n * sizeof(double), hipMemcpyDeviceToHost, c->stream));
-    if (nEvents)
-        DXMC_CHECK(c, hipMemcpyAsync(nEvents, t->nEvents, t->n * sizeof(uint32_t), hipMemcpyDeviceToHost, c->stream));
-    if (variance)
-        DXMC_CHECK(c, hipMemcpyAsync(variance, t->variance, t->n * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    // through the context's page-locked staging chunks (a direct copy into
+    // pageable memory ran at about 7 GB/s), or direct into page-locked memory
+    int st = DXMC_OK;
+    if (dose && (st = c->download(dose, t->dose, t->n * sizeof(double))))
+        return st;
+    if (nEvents && (st = c->download(nEvents, t->nEvents, t->n * sizeof(uint32_t))))
+        return st;
+    if (variance && (st = c->download(variance, t->variance, t->n * sizeof(double))))
+        return st;
     DXMC_CHECK(c, hipStreamSynchronize(c->stream));
     return DXMC_OK;
 }

@@ -154,9 +154,12 @@ protected:
             + vectormath::dot(&m_directionCosines[0], &m_directionCosines[3]);
         if (std::abs(ortho) > T { 0.001 })
             return false;
-        const auto [mn, mx] = std::minmax_element(m_materialIndex->cbegin(), m_materialIndex->cend());
-        const auto nm = m_materialMap.size();
-        return *mn < nm && *mx < nm;
+        // every index below the number of materials (unsigned: the largest
+        // decides; a branch-free maximum, about 1 ms per 1e7 voxels)
+        std::uint8_t mx = 0;
+        for (const std::uint8_t v : *m_materialIndex)
+            mx = v > mx ? v : mx;
+        return mx < m_materialMap.size();
     }
 
     std::array<T, 3> m_spacing = { 1, 1, 1 };

@@ -9,10 +9,12 @@
 #include "dxmc/transport.hpp"
 #include "dxmc/world.hpp"
 
+#include <algorithm>
 #include <cmath>
 #include <memory>
 #include <mutex>
 #include <string>
+#include <thread>
 
 using namespace dxmc;
 
@@ -195,6 +197,30 @@ std::unique_ptr<Source<double>> makeSource(const dxmc_source_spec& s)
     }
 }
 
+// dst[i] = src[i] over large arrays on several threads: one core streams
+// about 9 GB/s, so the FP64 density copy of a 240^3 world took 19 ms alone
+template <typename D, typename S>
+void parallelCopy(D* dst, const S* src, std::size_t n)
+{
+    constexpr std::size_t MIN_CHUNK = std::size_t { 1 } << 21;
+    const unsigned cores = std::max(1u, std::thread::hardware_concurrency());
+    const unsigned threads = static_cast<unsigned>(std::min<std::size_t>({ std::size_t { 8 }, cores, std::max<std::size_t>(1, n / MIN_CHUNK) }));
+    if (threads <= 1) {
+        std::copy(src, src + n, dst);
+        return;
+    }
+    const std::size_t per = (n + threads - 1) / threads;
+    std::vector<std::thread> pool;
+    for (unsigned t = 1; t < threads; ++t) {
+        const std::size_t b = std::min(n, t * per), e = std::min(n, b + per);
+        if (b < e)
+            pool.emplace_back([=] { std::copy(src + b, src + e, dst + b); });
+    }
+    std::copy(src, src + std::min(n, per), dst);
+    for (auto& th : pool)
+        th.join();
+}
+
 bool buildWorld(const dxmc_host_world& w, World<double>& world, std::string& why, const WorldArrays* arrays)
 {
     if (!w.material || !w.density || !w.material_names || w.n_materials == 0)
@@ -206,8 +232,10 @@ bool buildWorld(const dxmc_host_world& w, World<double>& world, std::string& why
     const std::size_t n = w.dims[0] * w.dims[1] * w.dims[2];
     auto dens = arrays ? arrays->density(n) : std::make_shared<std::vector<double>>();
     auto mat = arrays ? arrays->bytes(n) : std::make_shared<std::vector<std::uint8_t>>();
-    dens->assign(w.density, w.density + n);
-    mat->assign(w.material, w.material + n);
+    dens->resize(n);
+    parallelCopy(dens->data(), w.density, n);
+    mat->resize(n);
+    parallelCopy(mat->data(), w.material, n);
     world.setDensityArray(dens);
     world.setMaterialIndexArray(mat);
     if (w.measurement || arrays) {
