@@ -9,6 +9,7 @@
 #include <cstring>
 #include <map>
 #include <string>
+#include <thread>
 #include <tuple>
 #include <cmath>
 #include <type_traits>
@@ -20,6 +21,37 @@ namespace dxmc_backend {
 void registerPinned(const void* p, size_t bytes);
 bool unregisterPinned(const void* p);
 bool pinnedHost(const void* p, size_t bytes);
+}
+
+// Host copy of one staging chunk, on four threads from 4 MiB up: one core
+// copies 27-30 GB/s out of page-locked memory on the MI355X host, four
+// 80-100 GB/s, while the device-to-host DMA runs at 53-57 GB/s. A part
+// whose thread cannot start is copied here.
+inline void chunkCopy(void* dst, const void* src, size_t len)
+{
+    constexpr size_t PART_MIN = size_t { 1 } << 22;
+    constexpr int PARTS = 4;
+    if (len < PART_MIN) {
+        std::memcpy(dst, src, len);
+        return;
+    }
+    const size_t per = (len + PARTS - 1) / PARTS;
+    auto part = [=](int k) {
+        const size_t b = std::min(len, k * per), e = std::min(len, b + per);
+        std::memcpy(static_cast<char*>(dst) + b, static_cast<const char*>(src) + b, e - b);
+    };
+    std::thread helpers[PARTS - 1];
+    for (int k = 1; k < PARTS; ++k) {
+        try {
+            helpers[k - 1] = std::thread(part, k);
+        } catch (...) {
+            part(k);
+        }
+    }
+    part(0);
+    for (auto& h : helpers)
+        if (h.joinable())
+            h.join();
 }
 
 struct dxmc_ctx {
@@ -144,7 +176,7 @@ struct dxmc_ctx {
         auto drain = [&](int b) {
             if (int st = check(hipEventSynchronize(stageDone[b])))
                 return st;
-            std::memcpy(to + pendingOff[b], stage[b], pendingLen[b]);
+            chunkCopy(to + pendingOff[b], stage[b], pendingLen[b]);
             pendingLen[b] = 0;
             return 0;
         };
