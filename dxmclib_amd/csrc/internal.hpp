@@ -23,10 +23,10 @@ bool unregisterPinned(const void* p);
 bool pinnedHost(const void* p, size_t bytes);
 }
 
-// Host copy of one staging chunk, on four threads from 4 MiB up: one core
-// copies 27-30 GB/s out of page-locked memory on the MI355X host, four
-// 80-100 GB/s, while the device-to-host DMA runs at 53-57 GB/s. A part
-// whose thread cannot start is copied here.
+// Host copy into or out of one staging chunk, on four threads from 4 MiB
+// up: one core copies 27-30 GB/s out of page-locked memory on the MI355X
+// host, four 80-100 GB/s, while the device-to-host DMA runs at 53-57 GB/s.
+// A part whose thread cannot start is copied here.
 inline void chunkCopy(void* dst, const void* src, size_t len)
 {
     constexpr size_t PART_MIN = size_t { 1 } << 22;
@@ -152,7 +152,7 @@ struct dxmc_ctx {
             if (i >= 2)
                 if (int st = check(hipEventSynchronize(stageDone[b])))
                     return st;
-            std::memcpy(stage[b], from + off, len);
+            chunkCopy(stage[b], from + off, len);
             if (int st = check(hipMemcpyAsync(to + off, stage[b], len, hipMemcpyHostToDevice, stream)))
                 return st;
             if (int st = check(hipEventRecord(stageDone[b], stream)))
