@@ -74,6 +74,7 @@ def lib():
         L.oracle_lut_lookup.argtypes = [C.c_void_p, C.c_int, C.c_uint64, u32p, f64p, f64p, f64p]
         L.oracle_interact.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_uint64, u32p, C.c_double, C.c_uint64, C.c_uint64, f64p, f64p, i32p]
         L.oracle_enter_world.argtypes = [C.POINTER(WorldView), C.c_uint64, f64p, i32p, u64p]
+        L.oracle_compton_ia.argtypes = [f64p, C.c_double, C.c_uint64, C.c_uint64, C.c_uint64, f64p]
         L.oracle_sample_particles.argtypes = [C.c_void_p, C.POINTER(WorldView), C.c_void_p, C.c_uint64, C.c_uint64, C.c_uint64, f64p, i32p]
         L.oracle_transport.argtypes = [C.c_void_p, C.POINTER(WorldView), C.c_void_p, C.c_uint64, C.c_int, C.c_int, C.c_uint64, C.c_uint64,
                                        C.c_uint32, f64p, u32p, f64p, u64p]
@@ -137,6 +138,18 @@ def rng_streams(seed, first, n, draws):
     out = np.zeros(n * draws, dtype=np.uint32)
     lib().oracle_rng_streams(seed, first, n, draws, _p(out, C.c_uint32))
     return out.reshape(n, draws)
+
+
+def compton_ia(shells, energy, n, seed, first=0):
+    """Impulse-approximation Compton sampler (L=2) on an explicit shell table
+    of up to 12 (binding keV, electrons, J0) rows; photons start along +z.
+    Returns [n][5]: returned energy, E', direction."""
+    sh = np.zeros((12, 3))
+    sh[:len(shells)] = shells
+    sh = np.ascontiguousarray(sh, dtype=np.float64)
+    out = np.zeros(n * 5)
+    lib().oracle_compton_ia(_p(sh, C.c_double), energy, seed, first, n, _p(out, C.c_double))
+    return out.reshape(n, 5)
 
 
 def enter_world(world, particles):

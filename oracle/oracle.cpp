@@ -1187,6 +1187,33 @@ void oracle_interact(void* h, int L, int kind, std::uint64_t n, const std::uint3
         interactL<2>(t, kind, n, mat, fp, seed, first, part, eout, aout);
 }
 
+// comptonScatterNRC (transport.hpp:342-483) on an explicit shell table, the
+// harness of the impulse-approximation tests: shells3 [12][3] = binding
+// energy (keV), electrons, J0; photon n starts at `energy` along +z with
+// history stream first + n; out [n][5] = returned energy, E', direction.
+void oracle_compton_ia(const double* shells3, double energy, std::uint64_t seed, std::uint64_t first, std::uint64_t n, double* out)
+{
+    TablesT<double> t;
+    t.nMat = 1;
+    t.shells.assign(12, Shell {});
+    for (int i = 0; i < 12; ++i) {
+        t.shells[i].binding = shells3[3 * i];
+        t.shells[i].electrons = shells3[3 * i + 1];
+        t.shells[i].j0 = shells3[3 * i + 2];
+    }
+    for (std::uint64_t i = 0; i < n; ++i) {
+        Rng rng = historyRng(seed, first + i);
+        Photon<double> p {};
+        p.dir[2] = 1;
+        p.energy = energy;
+        p.weight = 1;
+        double* q = out + 5 * i;
+        q[0] = comptonIA(t, p, 0, rng);
+        q[1] = p.energy;
+        std::copy(p.dir, p.dir + 3, q + 2);
+    }
+}
+
 // transportParticleToWorld + indexFromPosition for given particles [n][8]
 void oracle_enter_world(const WorldView* w, std::uint64_t n, double* part, int* inside, std::uint64_t* index)
 {
