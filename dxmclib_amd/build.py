@@ -94,7 +94,8 @@ def build_host(verbose=False):
 # programs written against the C++ host API the way a user of the reference
 # API writes them (tests/cpp), linked to the two libraries
 PROGRAMS = {"layered_pencil": os.path.join(ROOT, "tests", "cpp", "layered_pencil.cpp"),
-            "progress_preview": os.path.join(ROOT, "tests", "cpp", "progress_preview.cpp")}
+            "progress_preview": os.path.join(ROOT, "tests", "cpp", "progress_preview.cpp"),
+            "filter_weights": os.path.join(ROOT, "tests", "cpp", "filter_weights.cpp")}
 PROGRAM_DIR = os.path.join(ROOT, "tests", "cpp", "bin")
 
 

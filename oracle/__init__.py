@@ -74,6 +74,8 @@ def lib():
         L.oracle_lut_lookup.argtypes = [C.c_void_p, C.c_int, C.c_uint64, u32p, f64p, f64p, f64p]
         L.oracle_interact.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_uint64, u32p, C.c_double, C.c_uint64, C.c_uint64, f64p, f64p, i32p]
         L.oracle_enter_world.argtypes = [C.POINTER(WorldView), C.c_uint64, f64p, i32p, u64p]
+        L.oracle_bowtie_weight.argtypes = [C.c_void_p, C.c_int, C.c_uint64, f64p, f64p]
+        L.oracle_heel_weight.argtypes = [C.c_void_p, C.c_int, C.c_uint64, f64p, f64p, f64p]
         L.oracle_compton_ia.argtypes = [f64p, C.c_double, C.c_uint64, C.c_uint64, C.c_uint64, f64p]
         L.oracle_sample_particles.argtypes = [C.c_void_p, C.POINTER(WorldView), C.c_void_p, C.c_uint64, C.c_uint64, C.c_uint64, f64p, i32p]
         L.oracle_transport.argtypes = [C.c_void_p, C.POINTER(WorldView), C.c_void_p, C.c_uint64, C.c_int, C.c_int, C.c_uint64, C.c_uint64,
@@ -267,6 +269,19 @@ class Oracle:
         a = np.ascontiguousarray(angles, dtype=np.float64)
         w = np.ascontiguousarray(weights, dtype=np.float64)
         return lib().oracle_add_bowtie(self.h, _p(a, C.c_double), _p(w, C.c_double), len(a))
+
+    def bowtie_weight(self, table, angles):
+        q = np.ascontiguousarray(angles, dtype=np.float64)
+        out = np.zeros(len(q))
+        lib().oracle_bowtie_weight(self.h, table, len(q), _p(q, C.c_double), _p(out, C.c_double))
+        return out
+
+    def heel_weight(self, table, angles, energies):
+        a = np.ascontiguousarray(angles, dtype=np.float64)
+        e = np.ascontiguousarray(energies, dtype=np.float64)
+        out = np.zeros(len(a))
+        lib().oracle_heel_weight(self.h, table, len(a), _p(a, C.c_double), _p(e, C.c_double), _p(out, C.c_double))
+        return out
 
     def add_source_tables(self, src, spectra):
         """Register source tables in the order of a product HostTables.source_arrays():

@@ -22,8 +22,18 @@
 //   attenuation LUT + majorant             exactness at nodes, log-log
 //                                          linearity, majorant >= rho*mu,
 //                                          lagged segment extrapolation
-//   Livermore / IA Compton, RITA Rayleigh  energy conservation, scatter factor
-//   (L=1,2)                                bounds, forced-vs-naive expectation
+//   Livermore Compton, RITA Rayleigh (L=1) chi-square against KN x S(q) and
+//                                          (1+cos^2) F^2(q); energy conservation
+//   IA Compton (L=2)                       pz recovered from (E, E', cos) against
+//                                          the analytic profile (KS), shell
+//                                          choice, Q9, Klein-Nishina limit,
+//                                          angle distribution
+//   L=2 fluorescence                       line probabilities in closed form
+//   bowtie / heel weights                  np.interp of |angle| (the reference's
+//                                          bowtie does not compile with
+//                                          libstdc++); heel index rules in
+//                                          closed form; product heel table =
+//                                          normalised tube spectrum
 //   Woodcock tracking                      uncollided transmission exp(-mu L),
 //                                          majorant independence, energy balance
 //   normalisation                          closed form
@@ -1109,6 +1119,21 @@ int oracle_add_bowtie(void* h, const double* angles, const double* weights, std:
     s.bowWeight.insert(s.bowWeight.end(), weights, weights + n);
     s.bowOffset.push_back(static_cast<std::uint32_t>(s.bowAngle.size()));
     return static_cast<int>(s.bowOffset.size() - 2);
+}
+
+// bowtie weight of table id at nq angles; heel weight of table id at n
+// (angle, energy) pairs
+void oracle_bowtie_weight(void* h, int id, std::uint64_t nq, const double* q, double* out)
+{
+    const auto& s = static_cast<Context*>(h)->sources;
+    for (std::uint64_t i = 0; i < nq; ++i)
+        out[i] = bowtie<double>(s, id, q[i]);
+}
+void oracle_heel_weight(void* h, int id, std::uint64_t n, const double* angle, const double* energy, double* out)
+{
+    const auto& s = static_cast<Context*>(h)->sources;
+    for (std::uint64_t i = 0; i < n; ++i)
+        out[i] = heel<double>(s, id, angle[i], energy[i]);
 }
 
 // ----------------------------------------------------------- lookups
