@@ -95,7 +95,9 @@ def build_host(verbose=False):
 # API writes them (tests/cpp), linked to the two libraries
 PROGRAMS = {"layered_pencil": os.path.join(ROOT, "tests", "cpp", "layered_pencil.cpp"),
             "progress_preview": os.path.join(ROOT, "tests", "cpp", "progress_preview.cpp"),
-            "filter_weights": os.path.join(ROOT, "tests", "cpp", "filter_weights.cpp")}
+            "filter_weights": os.path.join(ROOT, "tests", "cpp", "filter_weights.cpp"),
+            "copy_helpers": os.path.join(ROOT, "tests", "cpp", "copy_helpers.cpp"),
+            "threads_cancel": os.path.join(ROOT, "tests", "cpp", "threads_cancel.cpp")}
 PROGRAM_DIR = os.path.join(ROOT, "tests", "cpp", "bin")
 
 

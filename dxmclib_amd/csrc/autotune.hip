@@ -142,6 +142,7 @@ extern "C" int dxmc_hip_autotune(dxmc_ctx* c, const dxmc_world* w, const dxmc_ta
 {
     if (!c || !best)
         return c ? c->fail(DXMC_ERR_ARGUMENT, "autotune: NULL argument") : DXMC_ERR_ARGUMENT;
+    DXMC_LOCK(c);
     int st = validateRun(c, w, t, x, nExp, base, nullptr);
     if (st)
         return st;

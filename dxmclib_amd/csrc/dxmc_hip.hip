@@ -330,6 +330,9 @@ int dxmc_hip_ctx_destroy(dxmc_ctx* c)
 {
     if (!c)
         return DXMC_ERR_ARGUMENT;
+    // wait for calls of other threads that still hold the context
+    c->lock.lock();
+    c->lock.unlock();
     hipSetDevice(c->device);
     hipStreamSynchronize(c->stream);
     hipEventDestroy(c->ev0);
@@ -337,16 +340,25 @@ int dxmc_hip_ctx_destroy(dxmc_ctx* c)
     c->releaseStaging();
     hipFree(c->historyCounter);
     hipStreamDestroy(c->stream);
+    threadErrors().erase(c);
     delete c;
     return DXMC_OK;
 }
 
-const char* dxmc_hip_last_error(const dxmc_ctx* c) { return c ? c->error.c_str() : globalError(); }
+const char* dxmc_hip_last_error(const dxmc_ctx* c)
+{
+    if (!c)
+        return globalError();
+    const auto& errors = threadErrors();
+    const auto it = errors.find(c);
+    return it != errors.end() ? it->second.c_str() : "";
+}
 
 int dxmc_hip_mem_info(dxmc_ctx* c, uint64_t* freeBytes, uint64_t* totalBytes)
 {
     if (!c)
         return DXMC_ERR_ARGUMENT;
+    DXMC_LOCK(c);
     size_t f = 0, t = 0;
     DXMC_CHECK(c, hipSetDevice(c->device));
     DXMC_CHECK(c, hipMemGetInfo(&f, &t));
@@ -385,6 +397,7 @@ int dxmc_hip_synchronize(dxmc_ctx* c)
 {
     if (!c)
         return DXMC_ERR_ARGUMENT;
+    DXMC_LOCK(c);
     DXMC_CHECK(c, hipStreamSynchronize(c->stream));
     return DXMC_OK;
 }
@@ -397,6 +410,7 @@ int dxmc_hip_tables_upload(dxmc_ctx* c, const dxmc_tables_desc* d, dxmc_tables**
 {
     if (!c || !d || !out)
         return DXMC_ERR_ARGUMENT;
+    DXMC_LOCK(c);
     *out = nullptr;
     std::string why;
     if (!validateTables(*d, why))
@@ -417,6 +431,7 @@ int dxmc_hip_tables_destroy(dxmc_ctx* c, dxmc_tables* t)
 {
     if (!c || !t)
         return DXMC_ERR_ARGUMENT;
+    DXMC_LOCK(c);
     if (t->ctx != c)
         return c->fail(DXMC_ERR_STATE, "tables_destroy: tables belong to another context");
     std::erase_if(c->tuned, [t](const auto& kv) { return std::get<1>(kv.first) == t; });
@@ -434,6 +449,7 @@ int dxmc_hip_tally_create(dxmc_ctx* c, uint64_t n, dxmc_tally** out)
 {
     if (!c || !out || n == 0)
         return DXMC_ERR_ARGUMENT;
+    DXMC_LOCK(c);
     *out = nullptr;
     DXMC_CHECK(c, hipSetDevice(c->device));
     auto* t = new dxmc_tally();
@@ -461,6 +477,7 @@ int dxmc_hip_tally_destroy(dxmc_ctx* c, dxmc_tally* t)
 {
     if (!c || !t)
         return DXMC_ERR_ARGUMENT;
+    DXMC_LOCK(c);
     if (t->ctx != c)
         return c->fail(DXMC_ERR_STATE, "tally_destroy: tally belongs to another context");
     hipSetDevice(c->device);
@@ -478,6 +495,7 @@ int dxmc_hip_tally_reset(dxmc_ctx* c, dxmc_tally* t)
 {
     if (!c || !t || t->ctx != c)
         return DXMC_ERR_ARGUMENT;
+    DXMC_LOCK(c);
     DXMC_CHECK(c, hipSetDevice(c->device));
     hipLaunchKernelGGL(fillZero, dim3(gridFor(t->n, 256)), dim3(256), 0, c->stream, static_cast<VoxelScore*>(t->scores), t->n);
     DXMC_CHECK(c, hipGetLastError());
@@ -489,6 +507,7 @@ int dxmc_hip_tally_download(dxmc_ctx* c, const dxmc_tally* t, double* dose, uint
 {
     if (!c || !t || t->ctx != c)
         return DXMC_ERR_ARGUMENT;
+    DXMC_LOCK(c);
     DXMC_CHECK(c, hipSetDevice(c->device));
     if (!t->arraysCurrent) { // the sums as they stand
         if (int st = launchNormalize(c, t, nullptr, -1, 0, 0.0))
@@ -511,6 +530,7 @@ int dxmc_hip_tally_mip(dxmc_ctx* c, const dxmc_tally* t, const dxmc_world* w, in
 {
     if (!c || !t || !w || !image || axis < 0 || axis > 2)
         return c ? c->fail(DXMC_ERR_ARGUMENT, "tally_mip: NULL argument or axis not 0, 1 or 2") : DXMC_ERR_ARGUMENT;
+    DXMC_LOCK(c);
     if (t->ctx != c || w->ctx != c)
         return c->fail(DXMC_ERR_STATE, "tally_mip: tally or world belongs to another context");
     if (t->n != w->n)
@@ -530,6 +550,7 @@ int dxmc_hip_tally_normalize(dxmc_ctx* c, dxmc_tally* t, const dxmc_world* w, in
 {
     if (!c || !t || t->ctx != c)
         return DXMC_ERR_ARGUMENT;
+    DXMC_LOCK(c);
     if (mode != DXMC_OUTPUT_EV_PER_HISTORY && mode != DXMC_OUTPUT_DOSE)
         return c->fail(DXMC_ERR_ARGUMENT, "tally_normalize: unknown output mode");
     if (mode == DXMC_OUTPUT_DOSE && (!w || w->n != t->n))
@@ -552,8 +573,9 @@ int dxmc_hip_run(dxmc_ctx* c, const dxmc_world* w, const dxmc_tables* t, const d
 int dxmc_hip_run_monitored(dxmc_ctx* c, const dxmc_world* w, const dxmc_tables* t, const dxmc_exposure* exposures, uint64_t nExp,
     const dxmc_run_cfg* cfg, dxmc_tally* tally, const dxmc_run_monitor* monitor, dxmc_run_stats* stats)
 {
-    if (!tally)
+    if (!tally || !c)
         return c ? c->fail(DXMC_ERR_ARGUMENT, "run: NULL argument") : DXMC_ERR_ARGUMENT;
+    DXMC_LOCK(c);
     int st = validateRun(c, w, t, exposures, nExp, cfg, tally);
     if (st)
         return st;
@@ -620,6 +642,7 @@ int dxmc_hip_rng_streams(dxmc_ctx* c, uint64_t seed, uint64_t first, uint64_t n,
 {
     if (!c || !out || n == 0 || draws == 0)
         return DXMC_ERR_ARGUMENT;
+    DXMC_LOCK(c);
     DXMC_CHECK(c, hipSetDevice(c->device));
     DeviceBuffer buf(c);
     int st = buf.alloc(n * draws * sizeof(uint32_t));
@@ -638,6 +661,7 @@ int dxmc_hip_lut_lookup(dxmc_ctx* c, const dxmc_tables* t, int32_t precision, ui
 {
     if (!c || !t || !mat || !energy || !outAtt || !outMaj || n == 0 || t->ctx != c)
         return DXMC_ERR_ARGUMENT;
+    DXMC_LOCK(c);
     for (uint64_t i = 0; i < n; ++i)
         if (mat[i] >= t->nMaterials || !(energy[i] > 0))
             return c->fail(DXMC_ERR_ARGUMENT, "lut_lookup: material out of range or energy not positive");
@@ -666,6 +690,7 @@ int dxmc_hip_interact(dxmc_ctx* c, const dxmc_tables* t, int32_t precision, int3
 {
     if (!c || !t || !mat || !particles || !energyOut || !aliveOut || n == 0 || t->ctx != c || L < 0 || L > 2 || kind < 0 || kind > 4)
         return c ? c->fail(DXMC_ERR_ARGUMENT, "interact: bad argument") : DXMC_ERR_ARGUMENT;
+    DXMC_LOCK(c);
     for (uint64_t i = 0; i < n; ++i)
         if (mat[i] >= t->nMaterials || !(particles[i * 8 + 6] > 0))
             return c->fail(DXMC_ERR_ARGUMENT, "interact: material out of range or energy not positive");
@@ -704,6 +729,7 @@ int dxmc_hip_sample_particles(dxmc_ctx* c, const dxmc_world* w, const dxmc_table
 {
     if (!c || !w || !t || !x || !particles || !inside || n == 0 || w->ctx != c || t->ctx != c)
         return DXMC_ERR_ARGUMENT;
+    DXMC_LOCK(c);
     if (x->spectrum >= static_cast<int32_t>(t->nSpectra) || x->heel >= static_cast<int32_t>(t->nHeel) || x->bowtie >= static_cast<int32_t>(t->nBowtie))
         return c->fail(DXMC_ERR_ARGUMENT, "sample_particles: exposure refers to a missing table");
     DXMC_CHECK(c, hipSetDevice(c->device));
@@ -825,7 +851,7 @@ int runTransport(dxmc_ctx* c, const dxmc_world* w, const dxmc_tables* t, const d
     const World<double> w64 = deviceWorld<double>(w);
     const World<float> w32 = deviceWorld<float>(w);
     for (uint64_t hb = 0; hb < total; hb += chunk) {
-        if (cancel && *cancel) {
+        if (cancel && __atomic_load_n(cancel, __ATOMIC_ACQUIRE)) {
             hipStreamSynchronize(c->stream);
             return c->fail(DXMC_ERR_CANCELLED, "run: cancelled");
         }
@@ -849,7 +875,7 @@ int runTransport(dxmc_ctx* c, const dxmc_world* w, const dxmc_tables* t, const d
         while (done < nExp && rb.start[done + 1] <= he)
             ++done;
         if (exposuresDone)
-            *exposuresDone = done;
+            __atomic_store_n(exposuresDone, done, __ATOMIC_RELEASE);
         const bool last = he >= total;
         if (preview
             && (last || std::chrono::duration<double>(std::chrono::steady_clock::now() - lastPreview).count() >= monitor->preview_interval_seconds))

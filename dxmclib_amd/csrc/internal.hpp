@@ -3,11 +3,13 @@
 #pragma once
 #include "device/physics.hpp"
 #include "dxmc_hip.h"
+#include "hostcopy.hpp"
 
 #include <hip/hip_runtime.h>
 #include <algorithm>
 #include <cstring>
 #include <map>
+#include <mutex>
 #include <string>
 #include <thread>
 #include <tuple>
@@ -23,35 +25,12 @@ bool unregisterPinned(const void* p);
 bool pinnedHost(const void* p, size_t bytes);
 }
 
-// Host copy into or out of one staging chunk, on four threads from 4 MiB
-// up: one core copies 27-30 GB/s out of page-locked memory on the MI355X
-// host, four 80-100 GB/s, while the device-to-host DMA runs at 53-57 GB/s.
-// A part whose thread cannot start is copied here.
-inline void chunkCopy(void* dst, const void* src, size_t len)
-{
-    constexpr size_t PART_MIN = size_t { 1 } << 22;
-    constexpr int PARTS = 4;
-    if (len < PART_MIN) {
-        std::memcpy(dst, src, len);
-        return;
-    }
-    const size_t per = (len + PARTS - 1) / PARTS;
-    auto part = [=](int k) {
-        const size_t b = std::min(len, k * per), e = std::min(len, b + per);
-        std::memcpy(static_cast<char*>(dst) + b, static_cast<const char*>(src) + b, e - b);
-    };
-    std::thread helpers[PARTS - 1];
-    for (int k = 1; k < PARTS; ++k) {
-        try {
-            helpers[k - 1] = std::thread(part, k);
-        } catch (...) {
-            part(k);
-        }
-    }
-    part(0);
-    for (auto& h : helpers)
-        if (h.joinable())
-            h.join();
+struct dxmc_ctx;
+
+namespace dxmc_backend {
+// the calling thread's last failure message per context: with several
+// threads on one context, each reads its own error
+std::map<const dxmc_ctx*, std::string>& threadErrors();
 }
 
 struct dxmc_ctx {
@@ -61,13 +40,18 @@ struct dxmc_ctx {
     hipStream_t stream = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     void* historyCounter = nullptr; // u64 work counter of the persistent kernel
-    std::string error;
     // autotuner results per (world, tables, precision, low energy correction)
     std::map<std::tuple<const void*, const void*, int, int>, dxmc_run_cfg> tuned;
+    // Held by every entry point for the whole call: the staging chunks, the
+    // work counter, the autotuner map and the stream's order of a call's
+    // steps are per context, so calls from several threads on one context
+    // run one after the other (device work on the stream is ordered anyway).
+    // Recursive: entry points call one another.
+    std::recursive_mutex lock;
 
     int fail(int status, const std::string& msg)
     {
-        error = msg;
+        dxmc_backend::threadErrors()[this] = msg;
         return status;
     }
     // records a HIP error, returns non-zero on failure
@@ -75,8 +59,7 @@ struct dxmc_ctx {
     {
         if (e == hipSuccess)
             return 0;
-        error = std::string("HIP error: ") + hipGetErrorString(e);
-        return DXMC_ERR_DEVICE;
+        return fail(DXMC_ERR_DEVICE, std::string("HIP error: ") + hipGetErrorString(e));
     }
     // Device memory is stream-ordered (hipMallocAsync / hipFreeAsync on the
     // context's stream, from the device's pool, which keeps freed blocks):
@@ -89,8 +72,7 @@ struct dxmc_ctx {
         const hipError_t e = hipMallocAsync(&q, bytes ? bytes : 1, stream);
         if (e != hipSuccess) {
             hipGetLastError();
-            error = "device allocation of " + std::to_string(bytes) + " bytes failed";
-            return DXMC_ERR_MEMORY;
+            return fail(DXMC_ERR_MEMORY, "device allocation of " + std::to_string(bytes) + " bytes failed");
         }
         *p = static_cast<P*>(q);
         return DXMC_OK;
@@ -108,6 +90,8 @@ struct dxmc_ctx {
     static constexpr size_t STAGE_MIN = size_t { 1 } << 20;
     void* stage[2] = { nullptr, nullptr };
     hipEvent_t stageDone[2] = { nullptr, nullptr };
+    // copy a chunk into or out of the staging memory on several host threads
+    dxmc_backend::CopyHelpers helpers;
 
     bool staging()
     {
@@ -120,10 +104,12 @@ struct dxmc_ctx {
                 releaseStaging();
                 return false;
             }
+        helpers.start(); // without helpers the copies run on the calling thread
         return true;
     }
     void releaseStaging()
     {
+        helpers.stop();
         for (int i = 0; i < 2; ++i) {
             if (stage[i])
                 hipHostFree(stage[i]);
@@ -152,7 +138,7 @@ struct dxmc_ctx {
             if (i >= 2)
                 if (int st = check(hipEventSynchronize(stageDone[b])))
                     return st;
-            chunkCopy(stage[b], from + off, len);
+            helpers.copy(stage[b], from + off, len);
             if (int st = check(hipMemcpyAsync(to + off, stage[b], len, hipMemcpyHostToDevice, stream)))
                 return st;
             if (int st = check(hipEventRecord(stageDone[b], stream)))
@@ -176,7 +162,7 @@ struct dxmc_ctx {
         auto drain = [&](int b) {
             if (int st = check(hipEventSynchronize(stageDone[b])))
                 return st;
-            chunkCopy(to + pendingOff[b], stage[b], pendingLen[b]);
+            helpers.copy(to + pendingOff[b], stage[b], pendingLen[b]);
             pendingLen[b] = 0;
             return 0;
         };
@@ -201,6 +187,9 @@ struct dxmc_ctx {
         return DXMC_OK;
     }
 };
+
+// serialises the calls on one context (see dxmc_ctx::lock)
+#define DXMC_LOCK(ctx) std::lock_guard<std::recursive_mutex> dxmc_lock_((ctx)->lock)
 
 #define DXMC_CHECK(ctx, expr)                      \
     do {                                           \

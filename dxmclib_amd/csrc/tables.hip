@@ -17,21 +17,16 @@
 namespace dxmc_backend {
 
 namespace {
-    std::mutex gErrMutex;
-    std::string gErr;
+    // errors are kept per calling thread: a message read by one thread is
+    // never rewritten by another thread's call
+    thread_local std::string gErr;
+    thread_local std::map<const dxmc_ctx*, std::string> gCtxErr;
     constexpr double LOG2_10 = 3.321928094887362347870319429489390175864831393;
 }
 
-void setGlobalError(const std::string& s)
-{
-    std::lock_guard lock(gErrMutex);
-    gErr = s;
-}
-const char* globalError()
-{
-    std::lock_guard lock(gErrMutex);
-    return gErr.c_str();
-}
+void setGlobalError(const std::string& s) { gErr = s; }
+const char* globalError() { return gErr.c_str(); }
+std::map<const dxmc_ctx*, std::string>& threadErrors() { return gCtxErr; }
 
 // page-locked blocks from dxmc_hip_host_alloc: start -> size
 namespace {

@@ -409,6 +409,7 @@ int dxmc_hip_world_upload(dxmc_ctx* c, const dxmc_world_desc* d, dxmc_world** ou
 {
     if (!c || !d || !out)
         return DXMC_ERR_ARGUMENT;
+    DXMC_LOCK(c);
     *out = nullptr;
     const uint64_t n = d->dims[0] * d->dims[1] * d->dims[2];
     if (n == 0 || !d->material || (!d->density_f32 == !d->density_f64) || d->n_materials == 0 || d->n_materials > 256)
@@ -481,6 +482,7 @@ int dxmc_hip_world_destroy(dxmc_ctx* c, dxmc_world* w)
 {
     if (!c || !w)
         return DXMC_ERR_ARGUMENT;
+    DXMC_LOCK(c);
     if (w->ctx != c)
         return c->fail(DXMC_ERR_STATE, "world_destroy: world belongs to another context");
     std::erase_if(c->tuned, [w](const auto& kv) { return std::get<0>(kv.first) == w; });

@@ -11,6 +11,7 @@
 #include "dxmc/floating.hpp"
 
 #include <array>
+#include <atomic>
 #include <chrono>
 #include <cstdint>
 #include <cstdio>
@@ -46,16 +47,17 @@ public:
     std::uint64_t completedExposures() const { return m_done; }
     void setPrefixMessage(const std::string& msg) { m_message = msg; }
     const std::string& prefixMessage() const { return m_message; }
-    void exposureCompleted() { m_done = m_done + 1; }
+    void exposureCompleted() { ++m_done; }
     std::string getETA() const
     {
         const std::uint64_t done = m_done;
         if (done == 0)
             return m_message + "ETA: estimating... [0%]";
+        const std::uint64_t total = m_total;
         const double elapsed = std::chrono::duration<double>(std::chrono::steady_clock::now() - m_start).count();
-        const double remaining = elapsed / done * (m_total - done);
+        const double remaining = elapsed / done * (total - done);
         char buf[128];
-        std::snprintf(buf, sizeof(buf), "ETA: %.0f s [%.0f%%]", remaining, 100.0 * done / (m_total ? m_total : 1));
+        std::snprintf(buf, sizeof(buf), "ETA: %.0f s [%.0f%%]", remaining, 100.0 * done / (total ? total : 1));
         return m_message + buf;
     }
     void setCancel(bool c) { m_cancel = c ? 1 : 0; }
@@ -63,9 +65,10 @@ public:
     void setPlaneNormal(Axis a) { m_axis = a; }
     Axis planeNormal() const { return m_axis; }
 
-    // the device run loop writes / polls these
-    volatile std::uint64_t* exposureCounter() { return &m_done; }
-    const volatile std::int32_t* cancelFlag() const { return &m_cancel; }
+    // the device run loop writes / polls these through the C ABI, with atomic
+    // release stores and acquire loads of the same width as these atomics
+    volatile std::uint64_t* exposureCounter() { return reinterpret_cast<volatile std::uint64_t*>(&m_done); }
+    const volatile std::int32_t* cancelFlag() const { return reinterpret_cast<const volatile std::int32_t*>(&m_cancel); }
 
     // seconds between two device previews during a run (0: after every launch)
     void setPreviewInterval(double seconds) { m_previewInterval = seconds < 0 ? 0 : seconds; }
@@ -134,9 +137,11 @@ public:
     }
 
 private:
-    volatile std::uint64_t m_total = 0;
-    volatile std::uint64_t m_done = 0;
-    volatile std::int32_t m_cancel = 0;
+    static_assert(sizeof(std::atomic<std::uint64_t>) == sizeof(std::uint64_t) && std::atomic<std::uint64_t>::is_always_lock_free);
+    static_assert(sizeof(std::atomic<std::int32_t>) == sizeof(std::int32_t) && std::atomic<std::int32_t>::is_always_lock_free);
+    std::atomic<std::uint64_t> m_total { 0 };
+    std::atomic<std::uint64_t> m_done { 0 };
+    std::atomic<std::int32_t> m_cancel { 0 };
     std::chrono::steady_clock::time_point m_start = std::chrono::steady_clock::now();
     std::string m_message;
     Axis m_axis = Axis::Z;

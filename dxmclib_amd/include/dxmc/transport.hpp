@@ -326,7 +326,10 @@ protected:
             ok = ok && !dxmc_hip_run_monitored(c, dw, dt, prep.exposures.data(), prep.exposures.size(), &cfg, tally, &monitor, &m_stats);
         }
         if (ok && !(pb && pb->cancel())) {
-            ok = !(nHist > 1 && dxmc_hip_tally_normalize(c, tally, dw, mode, nHist, mode == DXMC_OUTPUT_DOSE ? calibration() : 1.0))
+            // dose does not depend on the history count (transport.hpp:
+            // 796-816); eV/history needs two or more histories (780-794)
+            const bool normalize = mode == DXMC_OUTPUT_DOSE || nHist > 1;
+            ok = !(normalize && dxmc_hip_tally_normalize(c, tally, dw, mode, nHist, mode == DXMC_OUTPUT_DOSE ? calibration() : 1.0))
                 && !dxmc_hip_synchronize(c);
             t2 = clock::now();
             ok = ok && !dxmc_hip_tally_download(c, tally, dose, nEvents, variance);

@@ -211,10 +211,16 @@ void parallelCopy(D* dst, const S* src, std::size_t n)
     }
     const std::size_t per = (n + threads - 1) / threads;
     std::vector<std::thread> pool;
+    pool.reserve(threads - 1);
     for (unsigned t = 1; t < threads; ++t) {
         const std::size_t b = std::min(n, t * per), e = std::min(n, b + per);
-        if (b < e)
+        if (b >= e)
+            continue;
+        try {
             pool.emplace_back([=] { std::copy(src + b, src + e, dst + b); });
+        } catch (...) { // a thread that cannot start: its part on this one
+            std::copy(src + b, src + e, dst + b);
+        }
     }
     std::copy(src, src + std::min(n, per), dst);
     for (auto& th : pool)

@@ -12,9 +12,12 @@
  *
  * Conventions: plain pointers and sizes, no C++ or torch types; every call
  * returns 0 (DXMC_OK) or a status code, with a message available from
- * dxmc_hip_last_error(). One host thread per context; contexts are
- * independent. Device objects (world, tables, tally) belong to the context
- * that created them and stay resident in HBM until destroyed.
+ * dxmc_hip_last_error(). Contexts are independent. Several host threads may
+ * call into one context: each call holds the context for its duration, so
+ * calls on one context run one after the other (use one context per thread
+ * for concurrency, as the request server does). Device objects (world,
+ * tables, tally) belong to the context that created them and stay resident
+ * in HBM until destroyed.
  */
 #ifndef DXMC_HIP_H
 #define DXMC_HIP_H
@@ -58,8 +61,9 @@ typedef struct dxmc_tally dxmc_tally;
 /* Create a context on HIP device 'device'. */
 int dxmc_hip_ctx_create(int device, dxmc_ctx** out);
 int dxmc_hip_ctx_destroy(dxmc_ctx* ctx);
-/* Last error message of the context (or of the last failed ctx_create when
- * ctx is NULL). The pointer stays valid until the next call on ctx. */
+/* Last error message of the calling thread's calls on the context ("" if
+ * none failed), or of its last failed ctx_create or host_alloc/free when
+ * ctx is NULL. The pointer stays valid until the thread's next call on ctx. */
 const char* dxmc_hip_last_error(const dxmc_ctx* ctx);
 int dxmc_hip_abi_version(void);
 /* Device memory in bytes: free and total. */
@@ -234,6 +238,9 @@ typedef struct {
  * imparted to the tally (not reset). cancel (may be NULL) is polled between
  * launches: a non-zero value stops the run with DXMC_ERR_CANCELLED.
  * exposures_done (may be NULL) receives the number of exposures finished.
+ * Both are accessed with atomic acquire loads and release stores, so another
+ * thread may set *cancel or read *exposures_done with atomic operations of
+ * the same width (C11 _Atomic, std::atomic<int32_t> / <uint64_t>).
  * Replaces parallellRun<L> (transport.hpp:765-778) and its workers. */
 int dxmc_hip_run(dxmc_ctx* ctx, const dxmc_world* world, const dxmc_tables* tables, const dxmc_exposure* exposures,
     uint64_t n_exposures, const dxmc_run_cfg* cfg, dxmc_tally* tally, const volatile int32_t* cancel,

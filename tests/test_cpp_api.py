@@ -24,6 +24,7 @@ from dxmclib_amd import backend, build, capi
 
 EXE = os.path.join(build.PROGRAM_DIR, "layered_pencil")
 PREVIEW_EXE = os.path.join(build.PROGRAM_DIR, "progress_preview")
+THREADS_EXE = os.path.join(build.PROGRAM_DIR, "threads_cancel")
 NAMES = ["Air, Dry (near sea level)", "Water, Liquid", "Al"]
 
 
@@ -115,3 +116,43 @@ def test_progress_bar_preview(axis):
     assert g["mid_run"] >= 1
     assert [g["width"], g["height"]] == plane
     assert g["max_pixel"] == 255
+
+
+@pytest.mark.gpu
+def test_cancel_returns_empty_result():
+    """ProgressBar::setCancel(true) from another thread during a run gives the
+    reference's cancelled result (progressbar.hpp:77-84, transport.hpp:
+    176-185): numberOfHistories 0 and all-zero arrays of the world's size."""
+    assert os.path.exists(THREADS_EXE), "build() first"
+    print("threads_cancel cancel", flush=True)
+    r = subprocess.run([THREADS_EXE, "cancel"], capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0, r.stderr
+    g = json.loads(r.stdout)
+    print(g, flush=True)
+    assert 1 <= g["exposures_at_cancel"] < g["total_exposures"] == 400
+    assert g["completed"] < g["total_exposures"]
+    assert g["size"] == 40 ** 3
+    res = g["result"]
+    assert res["histories"] == 0 and res["zero"]
+    assert res["dose"] == 0 and res["variance"] == 0 and res["n_events"] == 0
+
+
+@pytest.mark.gpu
+def test_two_threads_share_the_device_context():
+    """Two Transport calls on two threads at once (one shared device context,
+    each C ABI call holding it) give the results of the same calls run one
+    after the other: equal step and event counts and per-voxel event counts,
+    doses equal up to the order of FP64 atomic additions."""
+    assert os.path.exists(THREADS_EXE), "build() first"
+    print("threads_cancel concurrent", flush=True)
+    r = subprocess.run([THREADS_EXE, "concurrent"], capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0, r.stderr
+    g = json.loads(r.stdout)
+    print(g, flush=True)
+    for k in ("a", "b"):
+        c, s = g[f"concurrent_{k}"], g[f"serial_{k}"]
+        assert c["histories"] == s["histories"] > 0
+        assert c["steps"] == s["steps"] > 0 and c["events"] == s["events"] > 0
+        assert c["n_events"] == s["n_events"]
+        assert abs(c["dose"] - s["dose"]) <= 1e-12 * abs(s["dose"])
+        assert g[f"max_rel_{k}"] < 1e-12

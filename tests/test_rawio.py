@@ -40,3 +40,4 @@ def test_wrong_size_and_missing_file(tmp_path):
     assert st == capi.DXMC_ERR_ARGUMENT
     with pytest.raises(backend.DxmcError):
         backend._check_host(st)
+
