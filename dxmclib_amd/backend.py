@@ -40,7 +40,7 @@ def source_spec(type, **kw):
     s.heel_effect = 1
     keep = []
     for k, v in kw.items():
-        if k in ("bowtie_angles", "bowtie_weights", "spectrum_weights", "spectrum_energies"):
+        if k in ("bowtie_angles", "bowtie_weights", "spectrum_weights", "spectrum_energies", "bowtie_b_angles", "bowtie_b_weights"):
             a = np.ascontiguousarray(v, dtype=np.float64)
             keep.append(a)
             setattr(s, k, _ptr(a, C.c_double))
@@ -54,6 +54,8 @@ def source_spec(type, **kw):
         s.bowtie_n = len(kw["bowtie_angles"])
     if "spectrum_weights" in kw:
         s.spectrum_n = len(kw["spectrum_weights"])
+    if "bowtie_b_angles" in kw:
+        s.bowtie_b_n = len(kw["bowtie_b_angles"])
     s._keep = keep
     return s
 

@@ -36,6 +36,9 @@ SOURCE_DX = 3
 SOURCE_CBCT = 4
 SOURCE_CT_AXIAL = 5
 SOURCE_CT_SPIRAL = 6
+SOURCE_CT_AXIAL_DUAL = 7
+SOURCE_CT_SPIRAL_DUAL = 8
+SOURCE_CT_TOPOGRAM = 9
 
 u8p = C.POINTER(C.c_uint8)
 u32p = C.POINTER(C.c_uint32)
@@ -214,6 +217,25 @@ class SourceSpec(C.Structure):
         ("spectrum_weights", f64p),
         ("spectrum_energies", f64p),
         ("air_dose", C.c_double),
+        ("xcare", C.c_int32),
+        ("start_angle_b_set", C.c_int32),
+        ("xcare_filter_angle_deg", C.c_double),
+        ("xcare_span_angle_deg", C.c_double),
+        ("xcare_ramp_angle_deg", C.c_double),
+        ("xcare_low_weight", C.c_double),
+        ("tube_b_voltage", C.c_double),
+        ("tube_b_anode_angle_deg", C.c_double),
+        ("tube_b_al_mm", C.c_double),
+        ("tube_b_cu_mm", C.c_double),
+        ("tube_b_sn_mm", C.c_double),
+        ("sdd_b", C.c_double),
+        ("fov_b", C.c_double),
+        ("start_angle_b_deg", C.c_double),
+        ("mas_a", C.c_double),
+        ("mas_b", C.c_double),
+        ("bowtie_b_n", C.c_uint32),
+        ("bowtie_b_angles", f64p),
+        ("bowtie_b_weights", f64p),
     ]
 
 

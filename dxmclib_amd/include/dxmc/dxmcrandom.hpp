@@ -18,6 +18,7 @@
 #include <algorithm>
 #include <array>
 #include <cmath>
+#include <concepts>
 #include <cstdint>
 #include <random>
 #include <type_traits>
@@ -100,6 +101,23 @@ public:
         }
     }
 
+    // unsigned integer in [0, max) by rejection below (2^w - max) mod max, w
+    // the width of T: randomInteger for types up to 32 bits, and
+    // randomInteger32BitCapped for wider types whose values fit 32 bits
+    // (reference dxmcrandom.hpp:125-147)
+    template <std::unsigned_integral T>
+    inline T randomInteger(const T max) noexcept
+    {
+        static_assert(sizeof(T) <= 4, "randomInteger draws up to 32 bits; use randomInteger32BitCapped for wider types");
+        return rejectionInteger(max);
+    }
+    template <std::unsigned_integral T>
+    inline T randomInteger32BitCapped(const T max) noexcept
+    {
+        static_assert(sizeof(T) > 4, "randomInteger32BitCapped is for types wider than 32 bits; use randomInteger");
+        return rejectionInteger(max);
+    }
+
     inline std::uint32_t pcg32() noexcept
     {
         const std::uint64_t old = m_state[0];
@@ -109,6 +127,18 @@ public:
         return (xs >> rot) | (xs << ((0u - rot) & 31u));
     }
     std::uint64_t m_state[2];
+
+private:
+    template <typename T>
+    inline T rejectionInteger(const T max) noexcept
+    {
+        const T threshold = static_cast<T>(-max) % max;
+        for (;;) {
+            const T r = pcg32();
+            if (r >= threshold)
+                return r % max;
+        }
+    }
 };
 
 template <Floating T = double>

@@ -15,8 +15,10 @@
 #include <chrono>
 #include <cstdint>
 #include <cstdio>
+#include <iomanip>
 #include <memory>
 #include <mutex>
+#include <sstream>
 #include <string>
 #include <vector>
 

@@ -9,6 +9,9 @@
 //  CBCTSource         cone beam CT arc (DAP calibration)
 //  CTAxialSource      axial CT, one rotation per step (CTDIvol calibration)
 //  CTSpiralSource     helical CT (CTDIvol calibration)
+//  CTAxialDualSource  axial dual-source CT: two tubes, alternating exposures
+//  CTSpiralDualSource helical dual-source CT
+//  CTTopogramSource   CT scout: the tube fixed at the start angle, moving in z
 // CT calibration runs a nested transport in a CTDI phantom on the GPU.
 #pragma once
 
@@ -27,6 +30,7 @@
 #include <array>
 #include <cmath>
 #include <memory>
+#include <numeric>
 #include <vector>
 
 namespace dxmc {
@@ -464,7 +468,7 @@ public:
     const HeelFilter<T>* heelFilter() const { return m_heelFilter.get(); }
 
 protected:
-    void updateSpecterDistribution()
+    virtual void updateSpecterDistribution()
     {
         if (m_specterValid)
             return;
@@ -474,6 +478,35 @@ protected:
         m_heelFilter = m_modelHeelEffect ? std::make_shared<HeelFilter<T>>(m_tube, span) : nullptr;
         m_specterValid = true;
     }
+
+    // Gantry geometry of every CT exposure: the tube starts at (0, -SDD/2, 0)
+    // (the SDD of the source, also for a second tube), the gantry is tilted
+    // about x, the tube and the first cosine rotate about the (tilted) second
+    // cosine by 'angle', and the tube moves by zOffset along z.
+    void gantryPose(T angle, T zOffset, std::array<T, 3>& pos, std::array<T, 6>& c) const
+    {
+        pos = { 0, -m_sdd / T { 2 }, 0 };
+        c = this->m_directionCosines;
+        T* rotationAxis = &c[3];
+        T* otherAxis = &c[0];
+        const std::array<T, 3> tilt = { 1, 0, 0 };
+        auto tiltCorrection = pos;
+        vectormath::rotate(tiltCorrection.data(), tilt.data(), m_gantryTiltAngle);
+        vectormath::rotate(rotationAxis, tilt.data(), m_gantryTiltAngle);
+        vectormath::rotate(otherAxis, tilt.data(), m_gantryTiltAngle);
+        vectormath::rotate(pos.data(), rotationAxis, angle);
+        pos[2] += zOffset + tiltCorrection[2];
+        vectormath::rotate(otherAxis, rotationAxis, angle);
+        for (std::size_t i = 0; i < 3; ++i)
+            pos[i] += this->m_position[i];
+    }
+    std::array<T, 2> fanAngles(T fov, T sdd) const { return { std::atan(fov / sdd) * T { 2 }, std::atan(m_collimation / sdd) * T { 2 } }; }
+
+    // CTDIw from a nested transport in a CTDI phantom (forced scoring in
+    // the holes), reference source.hpp:925-988; U is CTAxialSource or
+    // CTAxialDualSource
+    template <typename U>
+    static T ctCalibration(U& copy, LOWENERGYCORRECTION model, ProgressBar<T>* progress);
 
     T m_sdd = 1190;
     T m_collimation = T { 38.4 };
@@ -513,37 +546,33 @@ public:
     virtual std::uint64_t exposuresPerRotatition() const { return static_cast<std::size_t>(2 * PI_VAL<T>() / m_exposureAngleStep); }
 
 protected:
-    // gantry geometry shared by axial and spiral exposures: the tube starts at
-    // (0, -SDD/2, 0), the gantry is tilted about x, the tube and the first
-    // cosine rotate about the (tilted) second cosine by 'angle'
-    Exposure<T> gantryExposure(T angle, T zOffset, std::uint64_t histories) const
+    // what an exposure takes from the tube that makes it
+    struct TubeSide {
+        T sdd, fov, weight;
+        const SpecterDistribution<T>* spectrum;
+        const HeelFilter<T>* heel;
+        const BeamFilter<T>* bowtie;
+    };
+    TubeSide tubeA() const
     {
-        std::array<T, 3> pos = { 0, -this->m_sdd / T { 2 }, 0 };
-        auto c = this->m_directionCosines;
-        T* rotationAxis = &c[3];
-        T* otherAxis = &c[0];
-        const std::array<T, 3> tilt = { 1, 0, 0 };
-        auto tiltCorrection = pos;
-        vectormath::rotate(tiltCorrection.data(), tilt.data(), this->m_gantryTiltAngle);
-        vectormath::rotate(rotationAxis, tilt.data(), this->m_gantryTiltAngle);
-        vectormath::rotate(otherAxis, tilt.data(), this->m_gantryTiltAngle);
-        vectormath::rotate(pos.data(), rotationAxis, angle);
-        pos[2] += zOffset + tiltCorrection[2];
-        vectormath::rotate(otherAxis, rotationAxis, angle);
-        for (std::size_t i = 0; i < 3; ++i)
-            pos[i] += this->m_position[i];
-        const std::array<T, 2> coll = { std::atan(this->m_fov / this->m_sdd) * T { 2 }, std::atan(this->m_collimation / this->m_sdd) * T { 2 } };
-        T weight { 1 };
+        return { this->m_sdd, this->m_fov, T { 1 }, this->m_specterDistribution.get(), this->m_heelFilter.get(), this->m_bowTieFilter.get() };
+    }
+    // an axial or spiral exposure at gantry angle 'angle' and z offset, with
+    // the AEC weight at the tube position and the XCare weight at the angle
+    Exposure<T> gantryExposure(T angle, T zOffset, std::uint64_t histories, const TubeSide& tube) const
+    {
+        std::array<T, 3> pos;
+        std::array<T, 6> c;
+        this->gantryPose(angle, zOffset, pos, c);
+        T weight = tube.weight;
         if (m_aecFilter)
             weight *= m_aecFilter->sampleIntensityWeight(pos);
         if (m_useXCareFilter)
             weight *= m_xcareFilter.sampleIntensityWeight(angle);
-        return Exposure<T>(pos, c, coll, histories, weight, this->m_specterDistribution.get(), this->m_heelFilter.get(), this->m_bowTieFilter.get());
+        return Exposure<T>(pos, c, this->fanAngles(tube.fov, tube.sdd), histories, weight, tube.spectrum, tube.heel, tube.bowtie);
     }
+    Exposure<T> gantryExposure(T angle, T zOffset, std::uint64_t histories) const { return gantryExposure(angle, zOffset, histories, tubeA()); }
 
-    // CTDIw from a nested transport in a CTDI phantom (forced scoring in
-    // the holes), reference source.hpp:925-988
-    static T ctCalibration(CTAxialSource<T>& copy, LOWENERGYCORRECTION model, ProgressBar<T>* progress);
 
     T m_exposureAngleStep = DEG_TO_RAD<T>();
     std::shared_ptr<AECFilter<T>> m_aecFilter;
@@ -588,7 +617,7 @@ public:
     T getCalibrationValue(LOWENERGYCORRECTION model, ProgressBar<T>* progress = nullptr) const override
     {
         auto copy = *this;
-        return CTSource<T>::ctCalibration(copy, model, progress);
+        return CTBaseSource<T>::ctCalibration(copy, model, progress);
     }
 
 private:
@@ -617,11 +646,250 @@ public:
     T getCalibrationValue(LOWENERGYCORRECTION model, ProgressBar<T>* progress = nullptr) const override
     {
         CTAxialSource<T> copy(*this);
-        return CTSource<T>::ctCalibration(copy, model, progress) * m_pitch;
+        return CTBaseSource<T>::ctCalibration(copy, model, progress) * m_pitch;
     }
 
 private:
     T m_pitch = 1;
+};
+
+// Dual-source CT (reference source.hpp:1079-1190): a second tube B on the
+// same gantry, a quarter turn ahead by default. Exposures alternate between
+// the tubes (even index A, odd index B); each tube has its own spectrum,
+// heel and bowtie filter, field of view, SDD and start angle, and its
+// exposures carry the weight 2 w_X / (w_A + w_B) with w_X = mAs_X times the
+// sum of tube X's unnormalised spectrum. The tube position follows the
+// source's SDD for both tubes, as in the reference.
+template <Floating T = double>
+class CTDualSource : public CTSource<T> {
+public:
+    CTDualSource()
+    {
+        this->m_type = Source<T>::Type::None;
+        m_sddB = this->m_sdd;
+        m_fovB = this->m_fov;
+        m_startAngleB = this->m_startAngle + PI_VAL<T>() * T { 0.5 };
+        m_tubeB.setAlFiltration(this->m_tube.AlFiltration());
+    }
+    T tubeAmas() const { return m_tubeAmas; }
+    T tubeBmas() const { return m_tubeBmas; }
+    void setTubeAmas(T mas)
+    {
+        this->m_specterValid = false;
+        m_tubeAmas = std::max(T { 0 }, mas);
+    }
+    void setTubeBmas(T mas)
+    {
+        this->m_specterValid = false;
+        m_tubeBmas = std::max(T { 0 }, mas);
+    }
+    Tube<T>& tubeB(void)
+    {
+        this->m_specterValid = false;
+        return m_tubeB;
+    }
+    const Tube<T>& tubeB(void) const { return m_tubeB; }
+    T maxPhotonEnergyProduced() const override { return std::max(this->m_tube.voltage(), m_tubeB.voltage()); }
+    std::uint64_t exposuresPerRotatition() const override { return 2 * static_cast<std::size_t>((2 * PI_VAL<T>()) / this->m_exposureAngleStep); }
+    void setBowTieFilterB(std::shared_ptr<BowTieFilter<T>> f) { m_bowTieFilterB = f; }
+    std::shared_ptr<BowTieFilter<T>> bowTieFilterB(void) { return m_bowTieFilterB; }
+    const std::shared_ptr<BowTieFilter<T>> bowTieFilterB() const { return m_bowTieFilterB; }
+    void setSourceDetectorDistanceB(T sdd)
+    {
+        this->m_specterValid = false;
+        m_sddB = std::abs(sdd);
+    }
+    T sourceDetectorDistanceB(void) const { return m_sddB; }
+    void setFieldOfViewB(T fov) { m_fovB = std::abs(fov); }
+    T fieldOfViewB(void) const { return m_fovB; }
+    void setStartAngleB(T a) { m_startAngleB = a; }
+    T startAngleB(void) const { return m_startAngleB; }
+    void setStartAngleDegB(T a) { m_startAngleB = DEG_TO_RAD<T>() * a; }
+    T startAngleDegB(void) const { return RAD_TO_DEG<T>() * m_startAngleB; }
+    // the exposure weights of tube A and B (2 w_X / (w_A + w_B), after validate)
+    T tubeAweight() const { return m_tubeAweight; }
+    T tubeBweight() const { return m_tubeBweight; }
+    const SpecterDistribution<T>* specterDistributionB() const { return m_specterDistributionB.get(); }
+    const HeelFilter<T>* heelFilterB() const { return m_heelFilterB.get(); }
+
+protected:
+    void updateSpecterDistribution() override
+    {
+        if (this->m_specterValid)
+            return;
+        const auto energyA = this->m_tube.getEnergy();
+        const auto energyB = m_tubeB.getEnergy();
+        auto specterA = this->m_tube.getSpecter(energyA, false);
+        auto specterB = m_tubeB.getSpecter(energyB, false);
+        const T sumA = std::accumulate(specterA.cbegin(), specterA.cend(), T { 0 });
+        const T sumB = std::accumulate(specterB.cbegin(), specterB.cend(), T { 0 });
+        const T weightA = m_tubeAmas * sumA;
+        const T weightB = m_tubeBmas * sumB;
+        for (auto& v : specterA)
+            v = v / sumA;
+        for (auto& v : specterB)
+            v = v / sumB;
+        m_tubeAweight = weightA * T { 2 } / (weightA + weightB);
+        m_tubeBweight = weightB * T { 2 } / (weightA + weightB);
+        this->m_specterDistribution = std::make_shared<SpecterDistribution<T>>(specterA, energyA);
+        m_specterDistributionB = std::make_shared<SpecterDistribution<T>>(specterB, energyB);
+        // both heel filters span tube A's collimation angle, as in the
+        // reference, and are kept regardless of the heel switch there
+        const T span = std::atan(this->m_collimation * T { 0.5 } / this->m_sdd) * T { 2 };
+        this->m_heelFilter = std::make_shared<HeelFilter<T>>(this->m_tube, span);
+        m_heelFilterB = std::make_shared<HeelFilter<T>>(m_tubeB, span);
+        this->m_specterValid = true;
+    }
+    // tube A for an even exposure index, tube B for an odd one, with its
+    // start angle
+    typename CTSource<T>::TubeSide tubeFor(std::uint64_t totalIndex, T& startAngle) const
+    {
+        if (totalIndex % 2 == 0) {
+            startAngle = this->m_startAngle;
+            return { this->m_sdd, this->m_fov, m_tubeAweight, this->m_specterDistribution.get(), this->m_heelFilter.get(),
+                this->m_bowTieFilter.get() };
+        }
+        startAngle = m_startAngleB;
+        return { m_sddB, m_fovB, m_tubeBweight, m_specterDistributionB.get(), m_heelFilterB.get(), m_bowTieFilterB.get() };
+    }
+
+    T m_sddB;
+    T m_fovB;
+    T m_startAngleB;
+    T m_tubeAmas = 100;
+    T m_tubeBmas = 100;
+    T m_tubeAweight = -1;
+    T m_tubeBweight = -1;
+    std::shared_ptr<BowTieFilter<T>> m_bowTieFilterB;
+    Tube<T> m_tubeB;
+    std::shared_ptr<SpecterDistribution<T>> m_specterDistributionB;
+    std::shared_ptr<HeelFilter<T>> m_heelFilterB;
+};
+
+template <Floating T>
+class CTSpiralDualSource;
+
+// axial dual-source CT (reference source.hpp:1369-1472): exposure pairs
+// (A, B) at each gantry step, one rotation per table step
+template <Floating T = double>
+class CTAxialDualSource final : public CTDualSource<T> {
+public:
+    CTAxialDualSource()
+    {
+        this->m_type = Source<T>::Type::CTDual;
+        m_step = this->m_collimation;
+        this->m_scanLenght = m_step;
+    }
+    CTAxialDualSource(const CTSpiralDualSource<T>& other);
+    Exposure<T> getExposure(std::uint64_t totalIndex) const override
+    {
+        T startAngle;
+        const auto tube = this->tubeFor(totalIndex, startAngle);
+        const std::uint64_t i = totalIndex / 2;
+        const std::uint64_t perRotation = static_cast<std::uint64_t>(2 * PI_VAL<T>() / this->m_exposureAngleStep);
+        const std::uint64_t rotation = i / perRotation;
+        const T angle = startAngle + this->m_exposureAngleStep * (i - (rotation * perRotation));
+        return this->gantryExposure(angle, m_step * rotation, this->m_historiesPerExposure, tube);
+    }
+    void setStep(T step)
+    {
+        const T n = this->m_scanLenght / m_step;
+        m_step = std::max(std::abs(step), T { 0.01 });
+        setScanLenght(m_step * n);
+    }
+    T step(void) const { return m_step; }
+    void setScanLenght(T l) override { this->m_scanLenght = std::max(m_step * std::ceil(std::abs(l) / m_step), m_step); }
+    std::uint64_t totalExposures(void) const override
+    {
+        const std::uint64_t perRotation = static_cast<std::uint64_t>(PI_VAL<T>() * T { 2 } / this->m_exposureAngleStep);
+        return perRotation * static_cast<std::uint64_t>(std::round(this->m_scanLenght / m_step)) * 2;
+    }
+    T getCalibrationValue(LOWENERGYCORRECTION model, ProgressBar<T>* progress = nullptr) const override
+    {
+        auto copy = *this;
+        return CTBaseSource<T>::ctCalibration(copy, model, progress);
+    }
+
+private:
+    T m_step = 10;
+};
+
+// helical dual-source CT (reference source.hpp:1474-1592)
+template <Floating T = double>
+class CTSpiralDualSource final : public CTDualSource<T> {
+public:
+    CTSpiralDualSource() { this->m_type = Source<T>::Type::CTDual; }
+    Exposure<T> getExposure(std::uint64_t totalIndex) const override
+    {
+        T startAngle;
+        const auto tube = this->tubeFor(totalIndex, startAngle);
+        const std::uint64_t i = totalIndex / 2;
+        const T angle = startAngle + this->m_exposureAngleStep * i;
+        constexpr T twoPi = T { 2 } * PI_VAL<T>();
+        const T z = (i * this->m_exposureAngleStep) * this->m_collimation * m_pitch / twoPi;
+        return this->gantryExposure(angle, z, this->m_historiesPerExposure, tube);
+    }
+    std::uint64_t totalExposures(void) const override
+    {
+        const auto single = static_cast<std::uint64_t>(this->m_scanLenght * 2 * PI_VAL<T>() / (this->m_collimation * m_pitch * this->m_exposureAngleStep));
+        return single * 2;
+    }
+    T getCalibrationValue(LOWENERGYCORRECTION model, ProgressBar<T>* progress = nullptr) const override
+    {
+        CTAxialDualSource<T> copy(*this);
+        return CTBaseSource<T>::ctCalibration(copy, model, progress) * m_pitch;
+    }
+    T pitch(void) const { return m_pitch; }
+    void setPitch(T p) { m_pitch = std::max(T { 0.01 }, p); }
+    void setScanLenght(T l) override { CTBaseSource<T>::setScanLenght(std::max(std::abs(l), this->m_collimation * m_pitch * T { 0.5 })); }
+
+private:
+    T m_pitch = 1;
+};
+
+template <Floating T>
+CTAxialDualSource<T>::CTAxialDualSource(const CTSpiralDualSource<T>& o)
+    : CTDualSource<T>(o)
+{
+    m_step = this->m_collimation;
+    setScanLenght(o.scanLenght());
+}
+
+// CT topogram (scout view; reference source.hpp:1609-1700): the tube stays
+// at the start angle and steps along z over the scan length, one exposure
+// per millimetre (at least one). Its dose calibration is that of an axial
+// scan of one collimation width holding the topogram's CTDIvol times scan
+// length over collimation, scaled by the ratio of exposure counts.
+template <Floating T = double>
+class CTTopogramSource final : public CTBaseSource<T> {
+public:
+    CTTopogramSource() { this->m_type = Source<T>::Type::CTTopogram; }
+    Exposure<T> getExposure(std::uint64_t i) const override
+    {
+        const std::uint64_t n = totalExposures();
+        // one exposure: z 0 (the reference divides by n - 1 = 0 there)
+        const T step = n > 1 ? this->scanLenght() / (n - 1) : T { 0 };
+        std::array<T, 3> pos;
+        std::array<T, 6> c;
+        this->gantryPose(this->m_startAngle, step * i, pos, c);
+        return Exposure<T>(pos, c, this->fanAngles(this->m_fov, this->m_sdd), this->m_historiesPerExposure, T { 1 }, this->m_specterDistribution.get(),
+            this->m_heelFilter.get(), this->m_bowTieFilter.get());
+    }
+    std::uint64_t totalExposures() const override { return std::max<std::uint64_t>(static_cast<std::uint64_t>(std::ceil(this->scanLenght())), 1); }
+    T getCalibrationValue(LOWENERGYCORRECTION model, ProgressBar<T>* progress = nullptr) const override
+    {
+        CTAxialSource<T> copy;
+        CTBaseSource<T>& base = copy;
+        base = *this;
+        copy.setCtdiVol(this->ctdiVol() * this->scanLenght() / this->collimation());
+        copy.setScanLenght(0);
+        copy.setStep(this->m_collimation);
+        // angle step matched to the number of exposures, at most 5 degrees
+        constexpr T maxStep = (2 * PI_VAL<T>()) / 72;
+        copy.setExposureAngleStep(std::min(2 * PI_VAL<T>() / totalExposures(), maxStep));
+        const T factor = CTBaseSource<T>::ctCalibration(copy, model, progress);
+        return (factor * totalExposures()) / copy.totalExposures();
+    }
 };
 
 template <Floating T>
@@ -634,7 +902,8 @@ CTAxialSource<T>::CTAxialSource(const CTSpiralSource<T>& o)
 }
 
 template <Floating T>
-T CTSource<T>::ctCalibration(CTAxialSource<T>& src, LOWENERGYCORRECTION model, ProgressBar<T>* progress)
+template <typename U>
+T CTBaseSource<T>::ctCalibration(U& src, LOWENERGYCORRECTION model, ProgressBar<T>* progress)
 {
     T meanWeight = 0;
     for (std::uint64_t i = 0; i < src.totalExposures(); ++i)

@@ -44,6 +44,29 @@ namespace vectormath {
     inline void cross(const T c[6], T res[3]) noexcept { cross(c, c + 3, res); }
 
     // Rodrigues rotation of v about a unit axis by angle (radians).
+    // angle between two vectors by the cancellation-free formula of
+    // W. Kahan ("Mindless" (2006) section 12) on the triangle a, b, a - b
+    template <Floating T>
+    inline T angleBetween(const T a[3], const T b[3]) noexcept
+    {
+        const T d[3] = { a[0] - b[0], a[1] - b[1], a[2] - b[2] };
+        const T la = lenght(a), lb = lenght(b), lc = lenght(d);
+        const T mu = lb >= lc ? lc - (la - lb) : lb - (la - lc);
+        return T { 2 } * std::atan(std::sqrt(((la - lb) + lc) * mu / ((la + (lb + lc)) * ((la - lc) + lb))));
+    }
+    // signed angle from v1 to v2 about the plane normal (all three are
+    // normalised in place, as in the reference)
+    template <Floating T>
+    inline T angleBetweenOnPlane(T v1[3], T v2[3], T planeNormal[3]) noexcept
+    {
+        normalize(v1);
+        normalize(v2);
+        normalize(planeNormal);
+        T c[3];
+        cross(v1, v2, c);
+        return std::atan2(dot(c, planeNormal), dot(v1, v2));
+    }
+
     template <Floating T>
     inline void rotate(T v[3], const T axis[3], const T angle) noexcept
     {

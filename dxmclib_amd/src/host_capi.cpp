@@ -79,14 +79,57 @@ void setupCT(S& src, const dxmc_source_spec& s)
         src.setFieldOfView(s.fov);
     src.setStartAngleDeg(s.start_angle_deg);
     src.setGantryTiltAngleDeg(s.gantry_tilt_deg);
-    if (s.exposure_angle_step_deg > 0)
-        src.setExposureAngleStepDeg(s.exposure_angle_step_deg);
+    if constexpr (requires { src.setExposureAngleStepDeg(1.0); }) // not the topogram
+        if (s.exposure_angle_step_deg > 0)
+            src.setExposureAngleStepDeg(s.exposure_angle_step_deg);
     if (s.ctdi_vol > 0)
         src.setCtdiVol(s.ctdi_vol);
     src.setModelHeelEffect(s.heel_effect != 0);
     if (s.bowtie_n > 0) {
         std::vector<double> a(s.bowtie_angles, s.bowtie_angles + s.bowtie_n), w(s.bowtie_weights, s.bowtie_weights + s.bowtie_n);
         src.setBowTieFilter(std::make_shared<BowTieFilter<double>>(a, w));
+    }
+}
+
+// XCare settings of CT axial, spiral and dual sources
+template <typename S>
+void setupXCare(S& src, const dxmc_source_spec& s)
+{
+    src.setUseXCareFilter(s.xcare != 0);
+    auto& f = src.xcareFilter();
+    if (s.xcare_filter_angle_deg != 0)
+        f.setFilterAngleDeg(s.xcare_filter_angle_deg);
+    if (s.xcare_span_angle_deg > 0)
+        f.setSpanAngleDeg(s.xcare_span_angle_deg);
+    if (s.xcare_ramp_angle_deg > 0)
+        f.setRampAngleDeg(s.xcare_ramp_angle_deg);
+    if (s.xcare_low_weight > 0)
+        f.setLowWeight(s.xcare_low_weight);
+}
+
+// tube B and the tube currents of a dual-source CT
+template <typename S>
+void setupDual(S& src, const dxmc_source_spec& s)
+{
+    dxmc_source_spec b = s;
+    b.tube_voltage = s.tube_b_voltage;
+    b.anode_angle_deg = s.tube_b_anode_angle_deg;
+    b.al_mm = s.tube_b_al_mm;
+    b.cu_mm = s.tube_b_cu_mm;
+    b.sn_mm = s.tube_b_sn_mm;
+    setupTube(src.tubeB(), b);
+    if (s.sdd_b > 0)
+        src.setSourceDetectorDistanceB(s.sdd_b);
+    if (s.fov_b > 0)
+        src.setFieldOfViewB(s.fov_b);
+    src.setStartAngleB(s.start_angle_b_set ? s.start_angle_b_deg * DEG_TO_RAD<double>() : src.startAngle() + PI_VAL<double>() / 2);
+    if (s.mas_a > 0)
+        src.setTubeAmas(s.mas_a);
+    if (s.mas_b > 0)
+        src.setTubeBmas(s.mas_b);
+    if (s.bowtie_b_n > 0) {
+        std::vector<double> a(s.bowtie_b_angles, s.bowtie_b_angles + s.bowtie_b_n), w(s.bowtie_b_weights, s.bowtie_b_weights + s.bowtie_b_n);
+        src.setBowTieFilterB(std::make_shared<BowTieFilter<double>>(a, w));
     }
 }
 
@@ -176,6 +219,7 @@ std::unique_ptr<Source<double>> makeSource(const dxmc_source_spec& s)
         auto p = std::make_unique<CTAxialSource<double>>();
         setupCommon(*p, s);
         setupCT(*p, s);
+        setupXCare(*p, s);
         if (s.axial_step > 0)
             p->setStep(s.axial_step);
         if (s.scan_length > 0)
@@ -186,8 +230,41 @@ std::unique_ptr<Source<double>> makeSource(const dxmc_source_spec& s)
         auto p = std::make_unique<CTSpiralSource<double>>();
         setupCommon(*p, s);
         setupCT(*p, s);
+        setupXCare(*p, s);
         if (s.pitch > 0)
             p->setPitch(s.pitch);
+        if (s.scan_length > 0)
+            p->setScanLenght(s.scan_length);
+        return p;
+    }
+    case DXMC_SOURCE_CT_AXIAL_DUAL: {
+        auto p = std::make_unique<CTAxialDualSource<double>>();
+        setupCommon(*p, s);
+        setupCT(*p, s);
+        setupXCare(*p, s);
+        setupDual(*p, s);
+        if (s.axial_step > 0)
+            p->setStep(s.axial_step);
+        if (s.scan_length > 0)
+            p->setScanLenght(s.scan_length);
+        return p;
+    }
+    case DXMC_SOURCE_CT_SPIRAL_DUAL: {
+        auto p = std::make_unique<CTSpiralDualSource<double>>();
+        setupCommon(*p, s);
+        setupCT(*p, s);
+        setupXCare(*p, s);
+        setupDual(*p, s);
+        if (s.pitch > 0)
+            p->setPitch(s.pitch);
+        if (s.scan_length > 0)
+            p->setScanLenght(s.scan_length);
+        return p;
+    }
+    case DXMC_SOURCE_CT_TOPOGRAM: {
+        auto p = std::make_unique<CTTopogramSource<double>>();
+        setupCommon(*p, s);
+        setupCT(*p, s);
         if (s.scan_length > 0)
             p->setScanLenght(s.scan_length);
         return p;

@@ -37,7 +37,10 @@ enum dxmc_source_type {
     DXMC_SOURCE_DX = 3,
     DXMC_SOURCE_CBCT = 4,
     DXMC_SOURCE_CT_AXIAL = 5,
-    DXMC_SOURCE_CT_SPIRAL = 6
+    DXMC_SOURCE_CT_SPIRAL = 6,
+    DXMC_SOURCE_CT_AXIAL_DUAL = 7, /* CTAxialDualSource, source.hpp:1369-1472 */
+    DXMC_SOURCE_CT_SPIRAL_DUAL = 8, /* CTSpiralDualSource, source.hpp:1474-1592 */
+    DXMC_SOURCE_CT_TOPOGRAM = 9 /* CTTopogramSource, source.hpp:1609-1700 */
 };
 
 /* Parameters of one source; fields not used by a type are ignored. A zero
@@ -74,6 +77,23 @@ typedef struct {
     const double* spectrum_energies;
     /* air dose (pencil, Gy cm2) */
     double air_dose;
+    /* XCare organ-based modulation of CT axial, spiral and dual sources
+     * (CTSource::setUseXCareFilter, XCareFilter setters; degrees, weight);
+     * a zero field keeps the filter's default */
+    int32_t xcare;
+    /* dual-source CT: 1 = tube B starts at start_angle_b_deg, 0 = at tube
+     * A's start angle + 90 degrees (the reference default) */
+    int32_t start_angle_b_set;
+    double xcare_filter_angle_deg, xcare_span_angle_deg, xcare_ramp_angle_deg, xcare_low_weight;
+    /* dual-source CT: tube B as the tube fields above (0 keeps the default,
+     * a negative filtration clears the filters; tube B starts with tube A's
+     * default 7 mm Al), its SDD, field of view and start angle, the tube
+     * currents (mAs, default 100 each) and tube B's bowtie */
+    double tube_b_voltage, tube_b_anode_angle_deg, tube_b_al_mm, tube_b_cu_mm, tube_b_sn_mm;
+    double sdd_b, fov_b, start_angle_b_deg, mas_a, mas_b;
+    uint32_t bowtie_b_n;
+    const double* bowtie_b_angles; /* rad */
+    const double* bowtie_b_weights;
 } dxmc_source_spec;
 
 /* Exposure table of a source aligned to a world basis (Exposure::

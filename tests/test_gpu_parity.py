@@ -177,6 +177,57 @@ def test_sample_particles(device, dev_tables, tables):
         device.destroy_world(w)
 
 
+def _tube_weights(kvp, al_mm):
+    """A dual-source tube's alias-table input: its unnormalised spectrum
+    (the oracle's literal integral) over the left-to-right sum."""
+    import math
+    import os
+    anode = dict(np.load(os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "anode_reference.npz")))
+    e = np.arange(1.0, kvp + 0.5, 1.0)
+    raw = oracle.tube_spectrum(anode, kvp, math.radians(12.0), e, [(13, al_mm)], normalize=False)
+    total = 0.0
+    for v in raw:
+        total += v
+    return raw / total, e
+
+
+def test_sample_particles_dual_source(device):
+    """Dual-source CT: exposures of tube A (even) and tube B (odd), each with
+    its own spectrum, heel and bowtie table, sampled on the device and by the
+    oracle from the same tables."""
+    n = 64
+    mat = np.zeros((n, n, n), dtype=np.uint8)
+    dens = np.ones((n, n, n), dtype=np.float32)
+    spacing = (2.0, 2.0, 2.0)
+    ext = backend.world_extent((n, n, n), spacing)
+    ang = np.linspace(0, 0.45, 32)
+    spec = backend.source_spec(capi.SOURCE_CT_AXIAL_DUAL, tube_voltage=120.0, al_mm=6.0, tube_b_voltage=80.0, tube_b_al_mm=4.0, sdd=1000.0,
+                               sdd_b=900.0, fov=500.0, fov_b=330.0, collimation=40.0, exposure_angle_step_deg=5.0, histories_per_exposure=1000,
+                               bowtie_angles=ang, bowtie_weights=np.exp(-6 * ang ** 2), bowtie_b_angles=ang, bowtie_b_weights=np.exp(-12 * ang ** 2),
+                               mas_a=100.0, mas_b=180.0)
+    t = backend.HostTables(["Water, Liquid"], [1.0], 120.0, 1.0)
+    exps = t.exposures(spec)
+    orc = oracle.Oracle(["Water, Liquid"], [1.0], 120.0, 1.0)
+    orc.add_source_tables(t.source_arrays(), [_tube_weights(120.0, 6.0), _tube_weights(80.0, 4.0)])
+    w = device.upload_world(mat, dens, spacing, ext, 1)
+    tt = device.upload_tables(t.desc())
+    try:
+        ow = oracle.world_view(mat, dens.astype(np.float64), spacing, ext)
+        for k in (0, 1, 10, 11):
+            print(f"dual exposure {k}", flush=True)
+            gp, gi = device.sample_particles(w, tt, capi.PRECISION_FP64, exps[k], 42, 1000 * k, 40000)
+            cp, ci = orc.sample_particles(ow, exps[k], 42, 1000 * k, 40000)
+            assert np.mean(gi == ci) >= 0.999
+            assert _agree(gp, cp) >= 0.999
+        # the tubes differ: tube B's photons are softer on average
+        ea = device.sample_particles(w, tt, capi.PRECISION_FP64, exps[0], 1, 0, 40000)[0][:, 6].mean()
+        eb = device.sample_particles(w, tt, capi.PRECISION_FP64, exps[1], 1, 0, 40000)[0][:, 6].mean()
+        assert eb < 0.9 * ea
+    finally:
+        device.destroy_tables(tt)
+        device.destroy_world(w)
+
+
 def _spectrum(spec):
     import ctypes as C
     n = C.c_uint64()
