@@ -150,7 +150,10 @@ struct World {
     const uint2* palette;
     const void* codes;
     const DensityBound* bounds;
-    uint32_t codeBytes, pad;
+    uint32_t codeBytes;
+    // 1 when the linear voxel index fits 24-bit multiply-adds: nz * ny and
+    // nx below 2^24 and the voxel count below 2^32
+    uint32_t index24;
 };
 
 // One voxel's accumulators in 32 aligned bytes: the three atomics of a

@@ -225,7 +225,7 @@ __global__ void fillZero(VoxelScore* scores, uint64_t n)
 uint32_t gridFor(uint64_t n, uint32_t block) { return static_cast<uint32_t>((n + block - 1) / block); }
 
 template <typename T>
-Exposure<T> toDevice(const dxmc_exposure& e)
+__host__ __device__ Exposure<T> toDevice(const dxmc_exposure& e)
 {
     Exposure<T> x;
     for (int i = 0; i < 3; ++i) {
@@ -804,8 +804,8 @@ int launchNormalize(dxmc_ctx* c, const dxmc_tally* t, const dxmc_world* w, int m
     return DXMC_OK;
 }
 
-int runTransport(dxmc_ctx* c, const dxmc_world* w, const dxmc_tables* t, const dxmc_exposure* x, uint64_t nExp, const dxmc_run_cfg& cfg,
-    dxmc_tally* tally, const dxmc_run_monitor* monitor, dxmc_run_stats* stats)
+int runLaunches(dxmc_ctx* c, const dxmc_world* w, const dxmc_tables* t, uint64_t nExp, const dxmc_run_cfg& cfg, dxmc_tally* tally,
+    const RunInputs& in, const dxmc_run_monitor* monitor, double& kernelMs, uint64_t& launches)
 {
     const volatile int32_t* cancel = monitor ? monitor->cancel : nullptr;
     volatile uint64_t* exposuresDone = monitor ? monitor->exposures_done : nullptr;
@@ -829,41 +829,31 @@ int runTransport(dxmc_ctx* c, const dxmc_world* w, const dxmc_tables* t, const d
         lastPreview = std::chrono::steady_clock::now();
         return DXMC_OK;
     };
-    DXMC_CHECK(c, hipSetDevice(c->device));
-    RunBuffers rb(c);
-    int st = uploadExposures(c, x, nExp, rb);
-    if (st)
-        return st;
-    const bool counters = (cfg.flags & 1u) != 0 || stats;
-    Counters* ctr = counters ? static_cast<Counters*>(rb.counters.p) : nullptr;
     const Tally tl { static_cast<VoxelScore*>(tally->scores) };
     tally->arraysCurrent = false;
-    const uint64_t total = rb.totalHistories;
     uint64_t chunk = cfg.histories_per_launch;
     if (chunk == 0)
         chunk = cfg.precision == DXMC_PRECISION_FP64 ? (uint64_t(1) << 22) : (uint64_t(1) << 27);
     const uint32_t block = cfg.block_size ? cfg.block_size : 256;
     if (block % 64 != 0 || block > 256)
         return c->fail(DXMC_ERR_ARGUMENT, "run: block size must be a multiple of 64 up to 256");
-    double kernelMs = 0;
-    uint64_t launches = 0;
+    kernelMs = 0;
+    launches = 0;
     uint64_t done = 0;
     const World<double> w64 = deviceWorld<double>(w);
     const World<float> w32 = deviceWorld<float>(w);
-    for (uint64_t hb = 0; hb < total; hb += chunk) {
+    for (uint64_t hb = 0; hb < in.total; hb += chunk) {
         if (cancel && __atomic_load_n(cancel, __ATOMIC_ACQUIRE)) {
             hipStreamSynchronize(c->stream);
             return c->fail(DXMC_ERR_CANCELLED, "run: cancelled");
         }
-        const uint64_t he = std::min(total, hb + chunk);
+        const uint64_t he = std::min(in.total, hb + chunk);
         DXMC_CHECK(c, hipEventRecord(c->ev0, c->stream));
         if (cfg.precision == DXMC_PRECISION_FP64)
-            launchPlainL<double>(cfg.low_energy_correction, w64, t->f64, static_cast<const Exposure<double>*>(rb.exp64.p),
-                static_cast<const uint64_t*>(rb.startDev.p), static_cast<uint32_t>(nExp), hb, he, cfg.seed, cfg.first_history, tl, ctr, block,
-                c->stream);
+            launchPlainL<double>(cfg.low_energy_correction, w64, t->f64, in.exp64, in.startDev, static_cast<uint32_t>(nExp), hb, he, cfg.seed,
+                cfg.first_history, tl, in.ctr, block, c->stream);
         else
-            launchFp32(c, cfg, w32, t->f32, static_cast<const Exposure<float>*>(rb.exp32.p), static_cast<const uint64_t*>(rb.startDev.p),
-                static_cast<uint32_t>(nExp), hb, he, tl, ctr);
+            launchFp32(c, cfg, w32, t->f32, in.exp32, in.startDev, static_cast<uint32_t>(nExp), hb, he, tl, in.ctr, in.historyCounter);
         DXMC_CHECK(c, hipGetLastError());
         DXMC_CHECK(c, hipEventRecord(c->ev1, c->stream));
         DXMC_CHECK(c, hipEventSynchronize(c->ev1));
@@ -872,21 +862,41 @@ int runTransport(dxmc_ctx* c, const dxmc_world* w, const dxmc_tables* t, const d
         kernelMs += ms;
         ++launches;
         // exposures whose last history is done
-        while (done < nExp && rb.start[done + 1] <= he)
-            ++done;
-        if (exposuresDone)
+        if (exposuresDone && in.startHost) {
+            while (done < nExp && in.startHost[done + 1] <= he)
+                ++done;
             __atomic_store_n(exposuresDone, done, __ATOMIC_RELEASE);
-        const bool last = he >= total;
+        }
+        const bool last = he >= in.total;
         if (preview
             && (last || std::chrono::duration<double>(std::chrono::steady_clock::now() - lastPreview).count() >= monitor->preview_interval_seconds))
             if (int e = sendPreview())
                 return e;
     }
+    return DXMC_OK;
+}
+
+int runTransport(dxmc_ctx* c, const dxmc_world* w, const dxmc_tables* t, const dxmc_exposure* x, uint64_t nExp, const dxmc_run_cfg& cfg,
+    dxmc_tally* tally, const dxmc_run_monitor* monitor, dxmc_run_stats* stats)
+{
+    DXMC_CHECK(c, hipSetDevice(c->device));
+    RunBuffers rb(c);
+    int st = uploadExposures(c, x, nExp, rb);
+    if (st)
+        return st;
+    const bool counters = (cfg.flags & 1u) != 0 || stats;
+    const RunInputs in { static_cast<const Exposure<double>*>(rb.exp64.p), static_cast<const Exposure<float>*>(rb.exp32.p),
+        static_cast<const uint64_t*>(rb.startDev.p), rb.start.data(), rb.totalHistories, counters ? static_cast<Counters*>(rb.counters.p) : nullptr,
+        static_cast<unsigned long long*>(c->historyCounter) };
+    double kernelMs = 0;
+    uint64_t launches = 0;
+    if ((st = runLaunches(c, w, t, nExp, cfg, tally, in, monitor, kernelMs, launches)))
+        return st;
     if (stats) {
         Counters h {};
         if (int e = c->download(&h, rb.counters.p, sizeof(Counters)))
-        return e;
-        stats->histories = total;
+            return e;
+        stats->histories = in.total;
         stats->steps = h.steps;
         stats->events = h.events;
         stats->kernel_ms = kernelMs;

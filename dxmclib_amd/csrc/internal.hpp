@@ -324,6 +324,7 @@ inline dxmc_dev::World<T> deviceWorld(const dxmc_world* w)
     d.codes = w->codes;
     d.bounds = static_cast<const dxmc_dev::DensityBound*>(w->bounds);
     d.codeBytes = w->codeBytes;
+    d.index24 = uint64_t(d.nz) * d.ny < (uint64_t(1) << 24) && d.nx < (uint32_t(1) << 24) && uint64_t(d.nx) * d.ny * d.nz < (uint64_t(1) << 32);
     d.voxelBytes = w->voxelBytes;
     return d;
 }
@@ -367,6 +368,23 @@ int autotune(dxmc_ctx* c, const dxmc_world* w, const dxmc_tables* t, const dxmc_
 uint32_t residentBlocksFp32(int L, uint32_t wavesPerSimd, uint32_t ldsPerBlock);
 int runTransport(dxmc_ctx* c, const dxmc_world* w, const dxmc_tables* t, const dxmc_exposure* x, uint64_t nExp, const dxmc_run_cfg& cfg,
     dxmc_tally* tally, const dxmc_run_monitor* monitor, dxmc_run_stats* stats);
+// device inputs of a run's launches: exposure records of both precisions,
+// history prefix sums (device; host copy for progress, may be null), total
+// histories, step/event counters (may be null) and the FP32 kernel's work
+// counter
+struct RunInputs {
+    const dxmc_dev::Exposure<double>* exp64;
+    const dxmc_dev::Exposure<float>* exp32;
+    const uint64_t* startDev;
+    const uint64_t* startHost;
+    uint64_t total;
+    dxmc_dev::Counters* ctr;
+    unsigned long long* historyCounter;
+};
+// the launch loop of a run on prepared inputs (cancel, progress and preview
+// through the monitor, which may be null)
+int runLaunches(dxmc_ctx* c, const dxmc_world* w, const dxmc_tables* t, uint64_t nExp, const dxmc_run_cfg& cfg, dxmc_tally* tally,
+    const RunInputs& in, const dxmc_run_monitor* monitor, double& kernelMs, uint64_t& launches);
 // maximum intensity projection of the tally's dose along axis into a device
 // image (launch only, on the context's stream); width and height returned
 int launchTallyMip(dxmc_ctx* c, const dxmc_tally* tally, const dxmc_world* w, int axis, float* image, uint64_t& width, uint64_t& height);
@@ -376,5 +394,5 @@ int launchNormalize(dxmc_ctx* c, const dxmc_tally* t, const dxmc_world* w, int m
 // FP32 transport launch (transport_fp32.hip)
 void launchFp32(dxmc_ctx* c, const dxmc_run_cfg& cfg, const dxmc_dev::World<float>& w, const dxmc_dev::Tables<float>& t,
     const dxmc_dev::Exposure<float>* exps, const uint64_t* expStart, uint32_t nExp, uint64_t hb, uint64_t he, const dxmc_dev::Tally& tally,
-    dxmc_dev::Counters* ctr);
+    dxmc_dev::Counters* ctr, unsigned long long* historyCounter);
 }

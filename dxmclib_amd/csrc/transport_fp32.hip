@@ -45,14 +45,15 @@ __device__ inline uint32_t rankBelow(uint64_t mask)
 }
 
 // voxel index with the reciprocal spacing (FP32 path), kept inside the grid
+// (the float-to-unsigned conversion clamps negative values to 0); with
+// 24-bit multiply-adds where the grid allows
 __device__ inline uint64_t voxelIndexFast(const World<float>& w, const float* pos, const float* inv)
 {
-    uint32_t ix = static_cast<uint32_t>((pos[0] - w.extent[0]) * inv[0]);
-    uint32_t iy = static_cast<uint32_t>((pos[1] - w.extent[2]) * inv[1]);
-    uint32_t iz = static_cast<uint32_t>((pos[2] - w.extent[4]) * inv[2]);
-    ix = ix < w.nx ? ix : w.nx - 1;
-    iy = iy < w.ny ? iy : w.ny - 1;
-    iz = iz < w.nz ? iz : w.nz - 1;
+    const uint32_t ix = min(static_cast<uint32_t>((pos[0] - w.extent[0]) * inv[0]), w.nx - 1);
+    const uint32_t iy = min(static_cast<uint32_t>((pos[1] - w.extent[2]) * inv[1]), w.ny - 1);
+    const uint32_t iz = min(static_cast<uint32_t>((pos[2] - w.extent[4]) * inv[2]), w.nz - 1);
+    if (w.index24)
+        return __umul24(__umul24(iz, w.ny) + iy, w.nx) + ix;
     return (static_cast<uint64_t>(iz) * w.ny + iy) * w.nx + ix;
 }
 
@@ -321,70 +322,86 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(W, 8))) voi
                 }
                 phase = alive ? FLYING : DEAD;
             }
-        } else if (phase == FLYING) {
-            // one Woodcock flight (woodcockParticleTracking, transport.hpp:
-            // 650-699), written without branches apart from the rare
-            // roulette: the random draws the reference makes only inside
-            // the world (decision) are drawn and kept by selection
-            const float u1 = rng.uniform<float>();
-            const float step = -__builtin_amdgcn_logf(u1) * (LN2F * 10.0f) * majInv; // cm -> mm
-            for (int i = 0; i < 3; ++i)
-                p.pos[i] += p.dir[i] * step;
-            const bool in = insideFast(w, p.pos);
-            idx = voxelIndexFast(w, p.pos, inv); // clamped: a valid voxel also outside
-            Rng r2 = rng;
-            const float u2 = r2.uniform<float>();
-            // The acceptance probability (Σμ·ρ)·(1/μ̂) is monotone in Σμ and
-            // in ρ. Bounds of both settle most draws exactly as the values
-            // would: u2 < p(lower bounds) accepts, u2 >= p(upper bounds)
-            // rejects. Σμ is bounded over the photon's LUT segment; ρ by the
-            // voxel's code when the world has codes, else read exactly. Only
-            // a draw between the two evaluates the attenuation (and reads the
-            // record).
-            float prob;
-            float dLo, dHi;
-            if (w.codeBytes != 0) {
-                const uint32_t code = w.codeBytes == 1 ? static_cast<const uint8_t*>(w.codes)[idx] : static_cast<const uint16_t*>(w.codes)[idx];
-                const DensityBound b = w.bounds[code];
-                mat = b.material;
-                meas = 0;
-                dLo = b.lo;
-                dHi = b.hi;
-                density = b.lo;
-            } else {
-                readVoxel(w, idx, mat, density, meas);
-                dLo = density;
-                dHi = density;
+        } else {
+            // Woodcock flights of the FLYING lanes, one after the other
+            // while no lane needs the outer loop: it ends when interactions
+            // are due (interaction_threshold pending lanes, or none flying)
+            // or dead lanes wait (for a refill of the queue while histories
+            // remain, or for regeneration). Only the flight's own values are
+            // live in this loop, so they stay in registers.
+            for (;;) {
+                if (phase == FLYING) {
+                    // one Woodcock flight (woodcockParticleTracking, transport.hpp:
+                    // 650-699), written without branches apart from the rare
+                    // roulette: the random draws the reference makes only inside
+                    // the world (decision) are drawn and kept by selection
+                    const float u1 = rng.uniform<float>();
+                    const float step = -__builtin_amdgcn_logf(u1) * (LN2F * 10.0f) * majInv; // cm -> mm
+                    for (int i = 0; i < 3; ++i)
+                        p.pos[i] += p.dir[i] * step;
+                    const bool in = insideFast(w, p.pos);
+                    idx = voxelIndexFast(w, p.pos, inv); // clamped: a valid voxel also outside
+                    Rng r2 = rng;
+                    const float u2 = r2.uniform<float>();
+                    // The acceptance probability (Σμ·ρ)·(1/μ̂) is monotone in Σμ and
+                    // in ρ. Bounds of both settle most draws exactly as the values
+                    // would: u2 < p(lower bounds) accepts, u2 >= p(upper bounds)
+                    // rejects. Σμ is bounded over the photon's LUT segment; ρ by the
+                    // voxel's code when the world has codes, else read exactly. Only
+                    // a draw between the two evaluates the attenuation (and reads the
+                    // record).
+                    float prob;
+                    float dLo, dHi;
+                    if (w.codeBytes != 0) {
+                        const uint32_t code = w.codeBytes == 1 ? static_cast<const uint8_t*>(w.codes)[idx] : static_cast<const uint16_t*>(w.codes)[idx];
+                        const DensityBound b = w.bounds[code];
+                        mat = b.material;
+                        meas = 0;
+                        dLo = b.lo;
+                        dHi = b.hi;
+                        density = b.lo;
+                    } else {
+                        readVoxel(w, idx, mat, density, meas);
+                        dLo = density;
+                        dHi = density;
+                    }
+                    bool settled = false;
+                    prob = 0.0f;
+                    if (t.lutSumBound) {
+                        const float2 sb = t.lutSumBound[static_cast<size_t>(mat) * t.lutNodes + lp.k];
+                        const float lo = sb.x * dLo * majInv;
+                        const float hi = sb.y * dHi * majInv;
+                        settled = (u2 < lo) | (u2 >= hi);
+                        prob = lo;
+                    }
+                    if (!settled) {
+                        if (w.codeBytes != 0)
+                            density = __uint_as_float(static_cast<const uint2*>(w.voxels)[idx].x);
+                        float att[3];
+                        attenuationAt(t, mat, lp, att);
+                        prob = ((att[0] + att[1]) + att[2]) * density * majInv;
+                    }
+                    const bool drawn = in & (meas == 0);
+                    rng.s = drawn ? r2.s : rng.s;
+                    const bool real = (meas != 0) | (u2 < prob);
+                    uint32_t next = in ? (real ? PENDING : FLYING) : DEAD;
+                    if (next == FLYING && p.energy * p.weight < 5.0f) { // Russian roulette
+                        if (rng.uniform<float>() < 0.8f)
+                            next = DEAD;
+                        else
+                            p.weight *= 1.0f / (1.0f - 0.8f);
+                    }
+                    steps += in ? 1u : 0u;
+                    historySteps += in ? 1u : 0u;
+                    phase = (next != DEAD && historySteps >= MAX_STEPS) ? DEAD : next;
+                }
+                const uint64_t flyNow = __ballot(phase == FLYING);
+                const uint32_t nPendNow = __popcll(__ballot(phase == PENDING));
+                const uint32_t nDeadNow = __popcll(__ballot(phase == DEAD));
+                if (flyNow == 0 || nPendNow >= interactThreshold || nDeadNow >= regenThreshold || (work && nDeadNow > queued)
+                    || (!work && nDeadNow > 0 && queued > 0))
+                    break;
             }
-            bool settled = false;
-            prob = 0.0f;
-            if (t.lutSumBound) {
-                const float2 sb = t.lutSumBound[static_cast<size_t>(mat) * t.lutNodes + lp.k];
-                const float lo = sb.x * dLo * majInv;
-                const float hi = sb.y * dHi * majInv;
-                settled = (u2 < lo) | (u2 >= hi);
-                prob = lo;
-            }
-            if (!settled) {
-                if (w.codeBytes != 0)
-                    density = __uint_as_float(static_cast<const uint2*>(w.voxels)[idx].x);
-                float att[3];
-                attenuationAt(t, mat, lp, att);
-                prob = ((att[0] + att[1]) + att[2]) * density * majInv;
-            }
-            const bool drawn = in & (meas == 0);
-            rng.s = drawn ? r2.s : rng.s;
-            const bool real = (meas != 0) | (u2 < prob);
-            uint32_t next = in ? (real ? PENDING : FLYING) : DEAD;
-            if (next == FLYING && p.energy * p.weight < 5.0f) { // Russian roulette
-                if (rng.uniform<float>() < 0.8f)
-                    next = DEAD;
-                else
-                    p.weight *= 1.0f / (1.0f - 0.8f);
-            }
-            steps += in ? 1u : 0u;
-            historySteps += in ? 1u : 0u;
-            phase = (next != DEAD && historySteps >= MAX_STEPS) ? DEAD : next;
         }
     }
     if (ctr)
@@ -396,6 +413,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(W, 8))) voi
 namespace dxmc_backend {
 
 namespace {
+
+__global__ void clearCounter(unsigned long long* counter) { *counter = 0; }
 
 using KernelFn = void (*)(World<float>, Tables<float>, const Exposure<float>*, const uint64_t*, uint32_t, uint64_t, uint64_t, uint64_t, uint64_t,
     Tally, Counters*, unsigned long long*, uint32_t, uint32_t, uint32_t);
@@ -468,10 +487,12 @@ uint32_t stageLevelFp32(const dxmc_run_cfg& cfg, const Tables<float>& t)
 }
 
 void launchFp32(dxmc_ctx* c, const dxmc_run_cfg& cfg, const World<float>& w, const Tables<float>& t, const Exposure<float>* exps,
-    const uint64_t* expStart, uint32_t nExp, uint64_t hb, uint64_t he, const Tally& tally, Counters* ctr)
+    const uint64_t* expStart, uint32_t nExp, uint64_t hb, uint64_t he, const Tally& tally, Counters* ctr, unsigned long long* historyCounter)
 {
-    auto* nx = static_cast<unsigned long long*>(c->historyCounter);
-    hipMemsetAsync(nx, 0, sizeof(unsigned long long), c->stream);
+    // the work counter is cleared by a kernel, not a memset, so the launch
+    // sequence stays valid under stream capture
+    auto* nx = historyCounter;
+    hipLaunchKernelGGL(clearCounter, dim3(1), dim3(1), 0, c->stream, nx);
     const uint32_t block = 256;
     const uint32_t waves = cfg.waves_per_simd;
     World<float> world = w;

@@ -77,6 +77,19 @@ def _ellipsoid(z, y, x, c, r):
     return ((x[None, None, :] - c[0]) / r[0]) ** 2 + ((y[None, :, None] - c[1]) / r[1]) ** 2 + ((z[:, None, None] - c[2]) / r[2]) ** 2 <= 1.0
 
 
+def _paint(mat, z, y, x, c, r, value):
+    """mat[_ellipsoid(z, y, x, c, r)] = value, evaluated only over the
+    ellipsoid's bounding box (the same voxels, without full-grid temporaries)."""
+    sl = []
+    for coord, cc, rr in ((z, c[2], r[2]), (y, c[1], r[1]), (x, c[0], r[0])):
+        idx = np.nonzero(np.abs(coord - cc) <= rr)[0]
+        if idx.size == 0:
+            return
+        sl.append(slice(int(idx[0]), int(idx[-1]) + 1))
+    sub = _ellipsoid(z[sl[0]], y[sl[1]], x[sl[2]], c, r)
+    mat[sl[0], sl[1], sl[2]][sub] = value
+
+
 def dx_thorax(histories_per_exposure=100_000, exposures=1000, seed=7):
     shape = (128, 256, 256)
     sp = (1.5, 1.5, 1.5)
@@ -98,7 +111,7 @@ def dx_thorax(histories_per_exposure=100_000, exposures=1000, seed=7):
             corners = [(c[0] + sx * r[0], c[1] + sy * r[1]) for sx in (-1, 1) for sy in (-1, 1)]
             if all((cx / 140.0) ** 2 + (cy / 90.0) ** 2 <= 1.0 for cx, cy in corners) and abs(c[2]) + r[2] < 96:
                 break
-        mat[_ellipsoid(z, y, x, c, r)] = 3 if k < 4 else 4
+        _paint(mat, z, y, x, c, r, 3 if k < 4 else 4)
     dens = np.asarray(dens_of, dtype=np.float32)[mat]
     spec = source_spec(capi.SOURCE_DX, tube_voltage=120.0, anode_angle_deg=12.0, energy_resolution=1.0, al_mm=2.5, sdd=1000.0,
                        field_size=(300.0, 300.0), histories_per_exposure=histories_per_exposure, total_exposures=exposures, heel_effect=1)
@@ -138,10 +151,10 @@ def ct_body(histories_total=1_000_000_000, seed=11, shape=(300, 512, 512), spaci
     for k in np.nonzero(z > -40)[0]:
         mat[k][shell] = 4
     for cx in (-65.0, 65.0):  # lungs
-        mat[_ellipsoid(z, y, x, (cx, -5.0, 40.0), (50.0, 65.0, 110.0))] = 3
+        _paint(mat, z, y, x, (cx, -5.0, 40.0), (50.0, 65.0, 110.0), 3)
     spine = ((x[None, :] / 18.0) ** 2 + ((y[:, None] - 70.0) / 18.0) ** 2 <= 1.0)
     mat[:, spine] = 4
-    mat[_ellipsoid(z, y, x, (40.0, 20.0, -70.0), (45.0, 35.0, 40.0))] = 5  # stomach (water)
+    _paint(mat, z, y, x, (40.0, 20.0, -70.0), (45.0, 35.0, 40.0), 5)  # stomach (water)
     rng = np.random.default_rng(seed)
     dens = dens_of[mat]
     dens *= rng.uniform(0.98, 1.02, size=shape).astype(np.float32)
@@ -166,12 +179,12 @@ def whole_body(histories_total=10_000_000_000, seed=13, shape=(600, 1024, 1024),
     mat[:] = s[None]
     rng = np.random.default_rng(seed)
     for cx in (-70.0, 70.0):
-        mat[_ellipsoid(z, y, x, (cx, -5.0, 60.0), (55.0, 70.0, 120.0))] = 3
+        _paint(mat, z, y, x, (cx, -5.0, 60.0), (55.0, 70.0, 120.0), 3)
     spine = ((x[None, :] / 20.0) ** 2 + ((y[:, None] - 75.0) / 20.0) ** 2 <= 1.0)
     mat[:, spine] = 4
-    mat[_ellipsoid(z, y, x, (45.0, 25.0, -60.0), (45.0, 35.0, 40.0))] = 5
+    _paint(mat, z, y, x, (45.0, 25.0, -60.0), (45.0, 35.0, 40.0), 5)
     for cx in (-60.0, 60.0):
-        mat[_ellipsoid(z, y, x, (cx, -95.0, 90.0), (40.0, 25.0, 40.0))] = 7
+        _paint(mat, z, y, x, (cx, -95.0, 90.0), (40.0, 25.0, 40.0), 7)
     dens = dens_of[mat] * rng.uniform(0.98, 1.02, size=shape).astype(np.float32)
     per = int(round(histories_total / 360))
     ang, w = bowtie_profile()
@@ -205,7 +218,7 @@ def cube_phantom(side, spacing, seed):
     half = side * spacing / 2
     r = rng.uniform(0.1, 0.3) * half
     c = rng.uniform(-half + r, half - r, 3)
-    mat[_ellipsoid(z, y, x, c, (r, r, r))] = 1
+    _paint(mat, z, y, x, c, (r, r, r), 1)
     dens = np.array([1.0, 1.92], dtype=np.float32)[mat]
     return mat, dens, [WATER, BONE]
 

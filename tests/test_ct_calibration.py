@@ -6,7 +6,9 @@ measurement holes, and CTDIvol / CTDIw / mean beam weight.
 CPU: the phantom geometry (disc, holes, measurement map).
 GPU: the calibration factor the host API applies in DOSE mode equals the
 factor restated here from the same phantom transported through the device C
-ABI (identical streams, FP64), and the centre/periphery ratio is physical.
+ABI (identical streams, FP64), and the centre/periphery ratio is physical;
+that transport of the CTDI phantom (forced scoring in the holes) equals the
+oracle's on identical streams at reduced histories.
 """
 import ctypes as C
 import math
@@ -105,3 +107,65 @@ def test_ct_calibration_factor(device):
     print(f"calibration factor {factor:.6e}, restated {expected:.6e}; centre/periphery {means[0] / np.mean(means[1:]):.3f}", flush=True)
     assert abs(factor - expected) <= 1e-9 * expected
     assert 0.25 < means[0] / np.mean(means[1:]) < 0.9
+
+
+@pytest.mark.gpu
+def test_ctdi_transport_against_oracle(device):
+    """The nested transport of the calibration (CT rotation at the phantom
+    centre, forced interactions in the measurement holes, source.hpp:925-988
+    on world.hpp:229-390) at reduced histories, FP64 device path against the
+    oracle on identical streams: totals, the five hole doses and the CTDIw
+    built from them agree, so the device doses the calibration factor comes
+    from are the oracle's."""
+    import oracle
+    per, step = 2000, 10.0
+    cspec = _ct_spec(per, step, 10.0, direction_cosines=(-1, 0, 0, 0, 0, 1), position=(0.0, 0.0, 0.0), scan_length=38.4)
+    names = ["Air, Dry (near sea level)", "Air, Dry (near sea level)", "Polymethyl Methacralate (Lucite, Perspex)"]
+    mat, dens, meas, hole, sp = _phantom(320)
+    ext = backend.world_extent(mat.shape[::-1], sp)
+    maxd = [float(dens[mat == i].max()) for i in range(3)]
+    t = backend.HostTables(names, maxd, 120.0, 1.0)
+    exps = t.exposures(cspec)
+    print(f"CTDI phantom: {len(exps)} exposures x {per} histories", flush=True)
+    w = device.upload_world(mat, dens, sp, ext, 3, measurement=meas)
+    tt = device.upload_tables(t.desc())
+    tally = device.tally(mat.size)
+    try:
+        stats = device.run(w, tt, exps, tally, capi.PRECISION_FP64, 1, seed=42)
+        d, _, v = device.download(tally, mat.size)
+    finally:
+        device.destroy_tally(tally)
+        device.destroy_tables(tt)
+        device.destroy_world(w)
+    spectrum = _spectrum(cspec)
+    orc = oracle.Oracle(names, maxd, 120.0, 1.0)
+    orc.add_source_tables(t.source_arrays(), [spectrum])
+    ow = oracle.world_view(mat, dens, sp, ext, measurement=meas)
+    ref = orc.transport(ow, exps, L=1, seed=42)
+    rel = abs(d.sum() - ref["dose"].sum()) / ref["dose"].sum()
+    flat = hole.ravel()
+    dev_holes = np.array([d[flat == h].mean() for h in range(1, 6)])
+    ref_holes = np.array([ref["dose"][flat == h].mean() for h in range(1, 6)])
+    hole_rel = np.abs(dev_holes / ref_holes - 1)
+    ctdiw = lambda m: (m[0] + 2 * np.mean(m[1:])) / 3 * 100.0 / 38.4  # noqa: E731
+    print(f"CTDI FP64 vs oracle: total rel {rel:.2e}, hole rel {hole_rel}, steps {stats.steps} vs {ref['steps']}, "
+          f"events {stats.events} vs {ref['events']}", flush=True)
+    assert rel < 5e-3
+    assert np.all(hole_rel < 5e-3)
+    assert abs(ctdiw(dev_holes) / ctdiw(ref_holes) - 1) < 2e-3
+    assert abs(stats.steps - ref["steps"]) <= 1e-3 * ref["steps"]
+    assert abs(stats.events - ref["events"]) <= 1e-3 * ref["events"]
+    # every hole scores (forced interactions), on the same voxels on both sides
+    for h in range(1, 6):
+        assert np.count_nonzero(ref["dose"][flat == h]) > 0
+    same = np.mean((d[flat > 0] > 0) == (ref["dose"][flat > 0] > 0))
+    assert same > 0.999
+
+
+def _spectrum(spec):
+    n = C.c_uint64()
+    lib = capi.host()
+    lib.dxmc_host_source_spectrum(C.byref(spec), None, None, 0, C.byref(n))
+    e, w = np.zeros(n.value), np.zeros(n.value)
+    lib.dxmc_host_source_spectrum(C.byref(spec), e.ctypes.data_as(C.POINTER(C.c_double)), w.ctypes.data_as(C.POINTER(C.c_double)), n.value, C.byref(n))
+    return w, e

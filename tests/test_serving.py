@@ -16,8 +16,8 @@ import pytest
 from dxmclib_amd import backend, capi, workloads
 
 
-def _request(side, kvp, histories, dx, seed):
-    req = {"side": side, "spacing": 1.5, "kvp": kvp, "histories": histories, "dx": dx, "seed": seed}
+def _request(side, kvp, histories, dx, seed, spacing=1.5):
+    req = {"side": side, "spacing": spacing, "kvp": kvp, "histories": histories, "dx": dx, "seed": seed}
     mat, dens, names = workloads.cube_phantom(side, req["spacing"], seed)
     world = backend.host_world(mat, dens, (req["spacing"],) * 3, names)
     return world, workloads.request_spec(req)
@@ -107,3 +107,29 @@ def test_server_matches_transport_call():
     direct = backend.transport(world, spec, precision=capi.PRECISION_FP32, output_mode=capi.OUTPUT_EV_PER_HISTORY, seed=5)
     assert served["info"].steps == direct["stats"].steps and served["info"].events == direct["stats"].events
     assert np.allclose(served["dose"], direct["dose"], rtol=1e-10, atol=0)
+
+
+@pytest.mark.gpu
+def test_full_size_request_same_alone_and_in_mix():
+    """The serving stream at its real size: a 256^3 CT-axial request with
+    1e8 histories (the stream's largest side and history count), alone and
+    admitted together with five requests of 128-224 voxels a side and 1e7
+    to 5e7 histories. Same histories (step and event counts, per-voxel
+    events) and dose up to the order of FP64 atomic additions."""
+    print("full-size request alone", flush=True)
+    target = _request(256, 120.0, 100_000_000, False, 51, spacing=1.2)
+    others = [_request(224, 140.0, 50_000_000, True, 52, spacing=1.8), _request(128, 80.0, 10_000_000, False, 53, spacing=2.0),
+              _request(192, 100.0, 30_000_000, True, 54, spacing=1.4), _request(160, 130.0, 20_000_000, False, 55, spacing=1.1),
+              _request(200, 90.0, 40_000_000, True, 56, spacing=1.6)]
+    with backend.Server(slots=4, prep_threads=8) as srv:
+        alone = _run_all(srv, [target], [9])[0]
+        print("full-size request in a mix", flush=True)
+        mix = _run_all(srv, others[:2] + [target] + others[2:], [61, 62, 9, 63, 64, 65])
+    got = mix[2]
+    a, b = alone["info"], got["info"]
+    print(f"alone: {a.histories} histories, steps {a.steps} events {a.events}; in mix: steps {b.steps} events {b.events}", flush=True)
+    assert a.histories == b.histories >= 99_000_000
+    assert a.steps == b.steps and a.events == b.events
+    assert np.array_equal(alone["n_events"], got["n_events"])
+    assert np.allclose(alone["dose"], got["dose"], rtol=1e-10, atol=0)
+    assert alone["dose"].size == 256 ** 3 and alone["dose"].sum() > 0
