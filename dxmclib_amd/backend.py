@@ -265,6 +265,18 @@ class Device:
                                           cancel, progress, C.byref(stats)))
         return stats
 
+    def run_noalloc(self, world, tables, exposures, tally, precision, low_energy_correction=1, seed=42, first_history=0, tuning=None):
+        """run() through dxmc_hip_run_noalloc (no allocation during the call;
+        at most capi.NOALLOC_MAX_EXPOSURES exposures)."""
+        cfg = tuning if tuning is not None else capi.RunCfg()
+        cfg.precision = precision
+        cfg.low_energy_correction = low_energy_correction
+        cfg.seed = seed
+        cfg.first_history = first_history
+        stats = capi.RunStats()
+        self._check(self.lib.dxmc_hip_run_noalloc(self.ctx, world, tables, exposures, len(exposures), C.byref(cfg), tally, C.byref(stats)))
+        return stats
+
     def run_monitored(self, world, tables, exposures, tally, precision, low_energy_correction=1, seed=42, tuning=None, preview=None,
                       preview_axis=2, preview_interval=0.0, cancel=None, progress=None):
         """run() with a live preview: preview(image) receives the projection

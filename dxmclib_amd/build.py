@@ -39,6 +39,7 @@ HIP_SOURCES = {
     "transport_fp32.hip": ["-fgpu-approx-transcendentals", "-fno-hip-fp32-correctly-rounded-divide-sqrt"],
     "autotune.hip": [],
     "world.hip": [],
+    "noalloc.hip": ["-ffp-contract=off"],
 }
 HOST_SOURCES = ["src/material.cpp", "src/host_capi.cpp", "src/server.cpp"]
 # -ftree-vectorize: integer passes over voxels (the material-index check)

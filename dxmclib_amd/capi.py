@@ -29,6 +29,8 @@ RUN_COUNTERS = 1
 RUN_AUTOTUNE = 2
 RUN_EXACT_STEPS = 4
 
+NOALLOC_MAX_EXPOSURES = 8192  # DXMC_NOALLOC_MAX_EXPOSURES
+
 SOURCE_PENCIL = 0
 SOURCE_ISOTROPIC = 1
 SOURCE_ISOTROPIC_CT = 2
@@ -329,6 +331,8 @@ HIP_SYMBOLS = {
                                C.POINTER(C.c_int32), C.POINTER(C.c_uint64), C.POINTER(RunStats)]),
     "dxmc_hip_run_monitored": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.POINTER(Exposure), C.c_uint64, C.POINTER(RunCfg), C.c_void_p,
                                          C.POINTER(RunMonitor), C.POINTER(RunStats)]),
+    "dxmc_hip_run_noalloc": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.POINTER(Exposure), C.c_uint64, C.POINTER(RunCfg), C.c_void_p,
+                                       C.POINTER(RunStats)]),
     "dxmc_hip_tally_mip": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int32, C.POINTER(C.c_float)]),
     "dxmc_hip_autotune": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.POINTER(Exposure), C.c_uint64, C.POINTER(RunCfg), C.c_uint64,
                                     C.POINTER(RunCfg), C.POINTER(TuneReport)]),

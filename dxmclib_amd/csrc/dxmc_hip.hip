@@ -309,6 +309,10 @@ int dxmc_hip_ctx_create(int device, dxmc_ctx** out)
         delete c;
         return DXMC_ERR_DEVICE;
     }
+    // the no-allocation run's scratch array: its address on this device,
+    // taken with the device current (a failure only disables that entry point)
+    if (noallocScratch(device, &c->scratch) != DXMC_OK)
+        c->scratch = nullptr;
     // keep memory freed by hipFreeAsync in the device pool for reuse instead
     // of returning it at every synchronisation
     hipMemPool_t pool = nullptr;

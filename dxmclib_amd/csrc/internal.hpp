@@ -40,6 +40,7 @@ struct dxmc_ctx {
     hipStream_t stream = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     void* historyCounter = nullptr; // u64 work counter of the persistent kernel
+    void* scratch = nullptr; // device address of the no-allocation run's scratch array (null: unavailable)
     // autotuner results per (world, tables, precision, low energy correction)
     std::map<std::tuple<const void*, const void*, int, int>, dxmc_run_cfg> tuned;
     // Held by every entry point for the whole call: the staging chunks, the
@@ -391,6 +392,9 @@ int launchTallyMip(dxmc_ctx* c, const dxmc_tally* tally, const dxmc_world* w, in
 // fills the tally's result arrays from its voxel scores: normalised for an
 // output mode, the sums as they stand for mode -1
 int launchNormalize(dxmc_ctx* c, const dxmc_tally* t, const dxmc_world* w, int mode, uint64_t nHist, double calibration);
+// device address of the no-allocation run's scratch array on the current
+// device (noalloc.hip), taken once per device
+int noallocScratch(int device, void** out);
 // FP32 transport launch (transport_fp32.hip)
 void launchFp32(dxmc_ctx* c, const dxmc_run_cfg& cfg, const dxmc_dev::World<float>& w, const dxmc_dev::Tables<float>& t,
     const dxmc_dev::Exposure<float>* exps, const uint64_t* expStart, uint32_t nExp, uint64_t hb, uint64_t he, const dxmc_dev::Tally& tally,

@@ -269,6 +269,24 @@ typedef struct {
 int dxmc_hip_run_monitored(dxmc_ctx* ctx, const dxmc_world* world, const dxmc_tables* tables, const dxmc_exposure* exposures,
     uint64_t n_exposures, const dxmc_run_cfg* cfg, dxmc_tally* tally, const dxmc_run_monitor* monitor, dxmc_run_stats* stats);
 
+/* dxmc_hip_run without allocation. No counterpart in the reference (its CPU
+ * workers allocate nothing per call either); for callers that cannot
+ * allocate device or host memory during a run: code under stream capture,
+ * tight serving loops, harnesses that forbid hipMalloc. The run's own device
+ * memory (exposure records, history prefix sums, counters) lives in a fixed
+ * scratch array of the library, one per GPU, held until the process ends
+ * (about 3.5 MB for DXMC_NOALLOC_MAX_EXPOSURES); world, tables and tally are
+ * the caller's, created beforehand. Limits: at most
+ * DXMC_NOALLOC_MAX_EXPOSURES exposures (more is DXMC_ERR_ARGUMENT); no
+ * cancel, progress or preview; DXMC_RUN_AUTOTUNE only reuses a configuration
+ * tuned before for the same world, tables, precision and model. Calls on one
+ * GPU share its array: the library runs them one after the other (the call
+ * returns when the run is done), so only calls from separate processes run
+ * at once. Same results as dxmc_hip_run for the same inputs. */
+#define DXMC_NOALLOC_MAX_EXPOSURES 8192
+int dxmc_hip_run_noalloc(dxmc_ctx* ctx, const dxmc_world* world, const dxmc_tables* tables, const dxmc_exposure* exposures, uint64_t n_exposures,
+    const dxmc_run_cfg* cfg, dxmc_tally* tally, dxmc_run_stats* stats);
+
 /* Autotuner of the transport kernels (no counterpart in the reference, whose
  * CPU threads need no tuning). Times trial launches of trial_histories
  * histories drawn evenly from the exposure table into a scratch tally and
