@@ -182,6 +182,10 @@ def serving(n_requests, rate, slots, prep_threads):
     lat = np.array([x.finished - x.submitted for x in infos])
     wait = np.array([x.admitted - x.prepared for x in infos])
     prep = np.array([x.prepared - x.submitted for x in infos])
+    prep_wait = np.array([x.prep_started - x.submitted for x in infos])
+    world_build = np.array([x.world_built - x.prep_started for x in infos])
+    source_build = np.array([x.source_ready - x.world_built for x in infos])
+    tables_build = np.array([x.prepared - x.source_ready for x in infos])
     run = np.array([x.finished - x.admitted for x in infos])
     upload = np.array([x.uploaded - x.admitted for x in infos])
     transport = np.array([x.transported - x.uploaded for x in infos])
@@ -195,12 +199,16 @@ def serving(n_requests, rate, slots, prep_threads):
     sides = [r["side"] for r in reqs]
     tail_phases = {"requests": int(tail.sum()), "voxels_mean": float(np.mean([sides[i] ** 3 for i in np.nonzero(tail)[0]])),
                    "histories_mean": float(np.mean([infos[i].histories for i in np.nonzero(tail)[0]])),
-                   "prepare_s": float(prep[tail].mean()), "admission_wait_s": float(wait[tail].mean()), "upload_s": float(upload[tail].mean()),
+                   "prepare_s": float(prep[tail].mean()), "prepare_wait_s": float(prep_wait[tail].mean()),
+                   "world_build_s": float(world_build[tail].mean()), "source_s": float(source_build[tail].mean()),
+                   "tables_s": float(tables_build[tail].mean()), "admission_wait_s": float(wait[tail].mean()), "upload_s": float(upload[tail].mean()),
                    "transport_s": float(transport[tail].mean()), "download_s": float(download[tail].mean()),
                    "release_s": float(release[tail].mean()), "publish_s": float(publish[tail].mean())}
     return {"requests": len(infos), "rate_per_s": rate, "slots": slots, "prep_threads": prep_threads, "p99_tail": tail_phases,
             "latency_p50_s": float(np.percentile(lat, 50)), "latency_p99_s": float(np.percentile(lat, 99)),
-            "latency_mean_s": float(lat.mean()), "prepare_mean_s": float(prep.mean()), "admission_wait_mean_s": float(wait.mean()),
+            "latency_mean_s": float(lat.mean()), "prepare_mean_s": float(prep.mean()), "prepare_wait_mean_s": float(prep_wait.mean()),
+            "world_build_mean_s": float(world_build.mean()), "source_mean_s": float(source_build.mean()), "tables_mean_s": float(tables_build.mean()),
+            "admission_wait_mean_s": float(wait.mean()),
             "device_mean_s": float(run.mean()), "upload_mean_s": float(upload.mean()), "transport_mean_s": float(transport.mean()),
             "download_mean_s": float(download.mean()), "release_mean_s": float(release.mean()),
             "publish_mean_s": float(publish.mean()), "histories": int(hist), "makespan_s": span, "histories_per_s": hist / span,
@@ -210,8 +218,9 @@ def serving(n_requests, rate, slots, prep_threads):
 def measured_traffic(workload, histories_per_launch):
     """HBM bytes per launch (GB) of the transport kernel from the committed
     rocprofv3 FETCH_SIZE / WRITE_SIZE passes of this workload (scaled by the
-    copy-kernel calibration), per history times this run's launch size; None
-    when no profile of the workload is committed."""
+    copy-kernel calibration), per history times this run's launch size, with
+    where the figure comes from; None when no profile of the workload is
+    committed."""
     path = os.path.join(ROOT, "profiles", "transport_fp32_ct_spiral.json")
     if not os.path.exists(path):
         return None
@@ -219,7 +228,8 @@ def measured_traffic(workload, histories_per_launch):
         prof = json.load(f)
     if prof.get("workload") != workload:
         return None
-    return (prof["fetch_bytes_per_history"] + prof["write_bytes_per_history"]) * histories_per_launch / 1e9
+    per = prof["fetch_bytes_per_history"] + prof["write_bytes_per_history"]
+    return per * histories_per_launch / 1e9, "rocprofv3 FETCH_SIZE+WRITE_SIZE per history (profiles/transport_fp32_ct_spiral.json) x histories per launch"
 
 
 def spectrum_of(spec):
@@ -319,8 +329,10 @@ def main():
             "data": "synthetic",
             "config": {"workload": wl.name, "histories": wl.histories, "exposures": len(exposures), "voxels": n,
                        "low_energy_correction": L, "description": wl.description},
+            # bound: the roofline the kernel is priced against (algorithmic HBM bytes); limiter: what the counters
+            # show holds it below that roof (vector-ALU issue, not bandwidth)
             "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK, "unit": "GB/s", "frac": achieved / HBM_PEAK,
-                         "traffic": None},
+                         "traffic": None, "traffic_unit": "GB per launch", "traffic_source": None, "limiter": "valu_issue"},
             "counters": {"steps_per_history": steps / histories, "events_per_history": events / histories,
                          "kernel_ms_per_launch": kernel_ms / max(launches, 1), "launches_per_step": launches / args.steps},
             "setup_seconds": setup_s,
@@ -330,7 +342,9 @@ def main():
         }
         result["roofline"]["achieved"] = achieved / 1e9
         result["roofline"]["peak"] = HBM_PEAK / 1e9
-        result["roofline"]["traffic"] = measured_traffic(wl.name, histories / max(launches, 1))
+        traffic = measured_traffic(wl.name, histories / max(launches, 1))
+        if traffic is not None:
+            result["roofline"]["traffic"], result["roofline"]["traffic_source"] = traffic
         if not args.no_fp64:
             sub = subset(exposures, args.fp64_stride)
             sub_hist = sum(x.n_histories for x in sub)

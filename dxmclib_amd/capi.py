@@ -297,6 +297,9 @@ class RequestInfo(C.Structure):
         ("transported", C.c_double),
         ("downloaded", C.c_double),
         ("released", C.c_double),
+        ("prep_started", C.c_double),
+        ("world_built", C.c_double),
+        ("source_ready", C.c_double),
     ]
 
 

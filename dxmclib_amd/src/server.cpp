@@ -67,6 +67,7 @@ struct Request {
     int32_t slot = -1;
     std::string error;
     double tSubmit = 0, tPrepared = 0, tAdmitted = 0, tUploaded = 0, tTransported = 0, tDownloaded = 0, tReleased = 0, tFinished = 0;
+    double tPrepStarted = 0, tWorldBuilt = 0, tSourceReady = 0;
 };
 
 // device memory a request holds while admitted: FP64 world (u8 material,
@@ -121,10 +122,16 @@ struct dxmc_server {
             }
             std::string why;
             int status = DXMC_OK;
-            if (!dxmc_host_detail::buildWorld(r->world, r->w, why, &worldArrays)) {
+            r->tPrepStarted = now();
+            const bool built = dxmc_host_detail::buildWorld(r->world, r->w, why, &worldArrays);
+            r->tWorldBuilt = now();
+            if (built && (r->source = dxmc_host_detail::makeSource(r->spec)))
+                r->source->validate(); // spectrum and filters (prepare() finds them valid)
+            r->tSourceReady = now();
+            if (!built) {
                 status = DXMC_ERR_ARGUMENT;
                 r->error = why;
-            } else if (!(r->source = dxmc_host_detail::makeSource(r->spec))) {
+            } else if (!r->source) {
                 status = DXMC_ERR_ARGUMENT;
                 r->error = "unknown source type";
             } else {
@@ -383,6 +390,9 @@ int dxmc_host_server_collect(dxmc_server* s, uint64_t id, double* dose, uint32_t
         info->transported = r->tTransported;
         info->downloaded = r->tDownloaded;
         info->released = r->tReleased;
+        info->prep_started = r->tPrepStarted;
+        info->world_built = r->tWorldBuilt;
+        info->source_ready = r->tSourceReady;
         info->admitted = r->tAdmitted;
         info->finished = r->tFinished;
         info->histories = r->stats.histories;

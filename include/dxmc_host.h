@@ -212,6 +212,12 @@ typedef struct {
      * device objects released (released); host objects released and the
      * result published at 'finished' */
     double uploaded, transported, downloaded, released;
+    /* preparation phases on the server clock: a preparation thread takes the
+     * request (prep_started), the World is built from the submitted arrays
+     * (world_built), the source's spectrum and filters are computed
+     * (source_ready); attenuation tables and exposure table end at
+     * 'prepared' */
+    double prep_started, world_built, source_ready;
 } dxmc_request_info;
 
 int dxmc_host_server_create(const dxmc_server_cfg* cfg, dxmc_server** out);
