@@ -151,9 +151,9 @@ struct World {
     const void* codes;
     const DensityBound* bounds;
     uint32_t codeBytes;
-    // 1 when the linear voxel index fits 24-bit multiply-adds: nz * ny and
-    // nx below 2^24 and the voxel count below 2^32
-    uint32_t index24;
+    // 1 when the linear voxel index fits 24-bit products: nx * ny (the
+    // slice stride 'nxy') and nz below 2^24 and the voxel count below 2^32
+    uint32_t index24, nxy;
 };
 
 // One voxel's accumulators in 32 aligned bytes: the three atomics of a
@@ -324,7 +324,8 @@ __device__ inline float lutValue(const float* c, const LutPoint<float>& p) { ret
 template <typename T>
 __device__ inline void attenuationAt(const Tables<T>& t, uint32_t mat, const LutPoint<T>& p, T att[3])
 {
-    const T* c = t.lutCoef + (static_cast<size_t>(mat) * t.lutNodes + p.k) * 6;
+    // row below 256 x 2^20: 24-bit product, 32-bit offset
+    const T* c = t.lutCoef + (__umul24(mat, t.lutNodes) + p.k) * 6u;
     att[0] = lutValue(c, p);
     att[1] = lutValue(c + 2, p);
     att[2] = lutValue(c + 4, p);

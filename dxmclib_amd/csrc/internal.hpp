@@ -325,7 +325,8 @@ inline dxmc_dev::World<T> deviceWorld(const dxmc_world* w)
     d.codes = w->codes;
     d.bounds = static_cast<const dxmc_dev::DensityBound*>(w->bounds);
     d.codeBytes = w->codeBytes;
-    d.index24 = uint64_t(d.nz) * d.ny < (uint64_t(1) << 24) && d.nx < (uint32_t(1) << 24) && uint64_t(d.nx) * d.ny * d.nz < (uint64_t(1) << 32);
+    d.index24 = uint64_t(d.nx) * d.ny < (uint64_t(1) << 24) && d.nz < (uint32_t(1) << 24) && uint64_t(d.nx) * d.ny * d.nz < (uint64_t(1) << 32);
+    d.nxy = d.index24 ? d.nx * d.ny : 0;
     d.voxelBytes = w->voxelBytes;
     return d;
 }

@@ -60,6 +60,9 @@ bool validateTables(const dxmc_tables_desc& d, std::string& why)
         return why = "need 1..256 materials", false;
     if (d.lut_nodes < 2 || !d.lut_grid || !d.lut_coefficients || !d.lut_majorant)
         return why = "attenuation LUT missing", false;
+    // LUT rows are addressed with 24-bit products and 32-bit offsets
+    if (d.lut_nodes >= (1u << 20))
+        return why = "attenuation LUT larger than 2^20 nodes", false;
     if (d.lut_linear_index > d.lut_nodes || !(d.lut_linear_step > 0))
         return why = "LUT linear index parameters invalid", false;
     for (uint32_t i = 1; i < d.lut_nodes; ++i)

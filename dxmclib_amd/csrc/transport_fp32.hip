@@ -46,14 +46,15 @@ __device__ inline uint32_t rankBelow(uint64_t mask)
 
 // voxel index with the reciprocal spacing (FP32 path), kept inside the grid
 // (the float-to-unsigned conversion clamps negative values to 0); with
-// 24-bit multiply-adds where the grid allows
+// 24-bit products where the grid allows (the compiler turns a 24-bit
+// multiply-add into a 64-bit one, so the three terms are summed instead)
 __device__ inline uint64_t voxelIndexFast(const World<float>& w, const float* pos, const float* inv)
 {
     const uint32_t ix = min(static_cast<uint32_t>((pos[0] - w.extent[0]) * inv[0]), w.nx - 1);
     const uint32_t iy = min(static_cast<uint32_t>((pos[1] - w.extent[2]) * inv[1]), w.ny - 1);
     const uint32_t iz = min(static_cast<uint32_t>((pos[2] - w.extent[4]) * inv[2]), w.nz - 1);
     if (w.index24)
-        return __umul24(__umul24(iz, w.ny) + iy, w.nx) + ix;
+        return __umul24(iz, w.nxy) + __umul24(iy, w.nx) + ix;
     return (static_cast<uint64_t>(iz) * w.ny + iy) * w.nx + ix;
 }
 
@@ -368,7 +369,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(W, 8))) voi
                     bool settled = false;
                     prob = 0.0f;
                     if (t.lutSumBound) {
-                        const float2 sb = t.lutSumBound[static_cast<size_t>(mat) * t.lutNodes + lp.k];
+                        const float2 sb = t.lutSumBound[__umul24(mat, t.lutNodes) + lp.k];
                         const float lo = sb.x * dLo * majInv;
                         const float hi = sb.y * dHi * majInv;
                         settled = (u2 < lo) | (u2 >= hi);
