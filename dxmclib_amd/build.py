@@ -32,11 +32,13 @@ HIP_FLAGS = [
 # per source: the FP64 plain path is compiled without FMA contraction so it
 # rounds like the CPU oracle; the FP32 transport kernel may contract and uses
 # the hardware transcendental instructions (v_exp_f32, v_log_f32, v_sin_f32,
-# v_cos_f32) and reciprocal-based division and square root.
+# v_cos_f32) and reciprocal-based division and square root, and flushes FP32
+# denormals (no FP32 value of the transport comes near 1e-38; with denormals
+# kept every division is a scaled reciprocal of about six instructions).
 HIP_SOURCES = {
     "dxmc_hip.hip": ["-ffp-contract=off"],
     "tables.hip": ["-ffp-contract=off"],
-    "transport_fp32.hip": ["-fgpu-approx-transcendentals", "-fno-hip-fp32-correctly-rounded-divide-sqrt"],
+    "transport_fp32.hip": ["-fgpu-approx-transcendentals", "-fno-hip-fp32-correctly-rounded-divide-sqrt", "-fgpu-flush-denormals-to-zero"],
     "autotune.hip": [],
     "world.hip": [],
     "noalloc.hip": ["-ffp-contract=off"],

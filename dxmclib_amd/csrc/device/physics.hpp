@@ -124,7 +124,7 @@ struct Exposure {
 
 // Voxel world. FP64 plain path: separate material / density / measurement
 // arrays as the reference. FP32 path: a record per distinct voxel content
-// (x = FP32 density bits, y = material | measurement << 8) in 'palette' and
+// (x = FP32 density bits, y = material | measurement << 24) in 'palette' and
 // a 1- or 2-byte palette index per voxel in 'voxels' (voxelBytes 1 or 2);
 // worlds with more than 65536 distinct records keep the 8-byte record itself
 // per voxel (voxelBytes 8, palette unused). Such a world may also carry a
@@ -234,8 +234,24 @@ __device__ inline void rotate(T* v, const T* axis, T angle)
     v[2] = r2;
 }
 
+// cosine and sine of a polar scattering angle
 template <typename T>
-__device__ inline void peturb(T* dir, T theta, T phi)
+struct Polar {
+    T c, s;
+};
+
+template <typename T>
+__device__ inline Polar<T> polarOfAngle(T theta) { return { cos(theta), sin(theta) }; }
+
+// The polar angle of a scatter from its sampled cosine. FP64 goes through
+// the angle as the reference does (acos, then cos and sin of it) and rounds
+// like it; FP32 keeps the cosine and takes the sine as sqrt(1 - c^2) (the
+// same values up to rounding: theta = acos(c) lies in [0, pi]).
+__device__ inline Polar<double> polarOfCos(double c) { return polarOfAngle(acos(c)); }
+__device__ inline Polar<float> polarOfCos(float c) { return { c, sqrt(fmaxf(0.0f, 1.0f - c * c)) }; }
+
+template <typename T>
+__device__ inline void peturb(T* dir, Polar<T> polar, T phi)
 {
     const T ax = fabs(dir[0]), ay = fabs(dir[1]), az = fabs(dir[2]);
     const int k = ax <= ay ? (ax <= az ? 0 : 2) : (ay <= az ? 1 : 2);
@@ -246,11 +262,17 @@ __device__ inline void peturb(T* dir, T theta, T phi)
     o[1] = dir[2] * e[0] - dir[0] * e[2];
     o[2] = dir[0] * e[1] - dir[1] * e[0];
     rotate(o, dir, phi);
-    const T st = sin(theta);
-    const T ct = cos(theta);
+    const T st = polar.s;
+    const T ct = polar.c;
     dir[0] = dir[0] * ct + o[0] * st;
     dir[1] = dir[1] * ct + o[1] * st;
     dir[2] = dir[2] * ct + o[2] * st;
+}
+
+template <typename T>
+__device__ inline void peturb(T* dir, T theta, T phi)
+{
+    peturb(dir, polarOfAngle(theta), phi);
 }
 
 // ------------------------------------------------------ attenuation tables
@@ -302,9 +324,10 @@ __device__ inline LutPoint<float> lutPoint(const Tables<float>& t, float E)
     // first guess in float (the linear index, or a search of the anchors)
     uint32_t k;
     if (le2 > t.lutLinearEnergy2) {
+        // f > 0; integer clamps (no float copy of lutNodes held in a
+        // register): from f >= lutNodes on, the last segment
         const float f = (le2 - t.lutLinearEnergy2) * t.lutLinearStepInv2;
-        k = f < float(t.lutNodes) ? static_cast<uint32_t>(f) + t.lutLinearIndex : t.lutNodes - 1;
-        k = k < t.lutNodes - 1 ? k : t.lutNodes - 1;
+        k = min(min(static_cast<uint32_t>(f), t.lutNodes) + t.lutLinearIndex, t.lutNodes - 1);
     } else {
         k = upperBound(t.lutAnchor, t.lutNodes, le2);
         k = k != t.lutNodes ? k : t.lutNodes - 1;
@@ -561,185 +584,233 @@ __device__ inline T photoAbsorption(const Tables<T>& t, Particle<T>& p, uint32_t
     }
 }
 
-template <typename T, int L>
-__device__ inline void rayleighScatter(const Tables<T>& t, Particle<T>& p, uint32_t mat, Rng& rng)
-{
-    if constexpr (L == 0) {
-        constexpr T extreme = (4 * Consts<T>::SQRT2) / (3 * Consts<T>::SQRT3);
-        bool reject = true;
-        T theta;
-        uint32_t trials = 0;
-        while (reject && trials++ < MAX_TRIALS) {
-            const T r1 = rng.uniform<T>(T(0), extreme);
-            theta = rng.uniform<T>(T(0), Consts<T>::PI);
-            const T s = sin(theta);
-            reject = r1 > ((2 - s * s) * s);
-        }
-        const T phi = rng.uniform<T>(Consts<T>::PI + Consts<T>::PI);
-        peturb(p.dir, theta, phi);
-    } else {
-        constexpr T kinv = T(1) / T(KEV_TO_ANGSTROM);
-        const T qmax = p.energy * kinv;
-        const T qmax2 = qmax * qmax;
-        T cosAngle;
-        uint32_t trials = 0;
-        do {
-            const T q2 = ritaSample(t, mat, rng, qmax2);
-            const T invE = T(KEV_TO_ANGSTROM) / p.energy;
-            cosAngle = 1 - 2 * q2 * invE * invE;
-        } while ((1 + cosAngle * cosAngle) * T(0.5) < rng.uniform<T>() && ++trials < MAX_TRIALS);
-        const T theta = acos(cosAngle);
-        const T phi = rng.uniform<T>(Consts<T>::PI + Consts<T>::PI);
-        peturb(p.dir, theta, phi);
-    }
-}
-
+// The Compton and Rayleigh samplers as a set-up (values that depend on the
+// photon only, and for the impulse approximation the shell drawn first) and
+// one attempt of their rejection loops (its draws and the acceptance test).
+// scatterEvent() below runs attempts until one is accepted or MAX_TRIALS are
+// spent, as the reference's loops do; the FP32 kernel runs a bounded number
+// of attempts per interaction pass and keeps the count between passes,
+// which draws the same numbers in the same order.
 template <typename T>
-__device__ inline T comptonImpulse(const Tables<T>& t, Particle<T>& p, uint32_t mat, Rng& rng)
+struct ScatterSetup {
+    T k, emin, gmaxInv; // Compton: E / mc^2, smallest kept energy fraction, 1 / largest KN weight
+    T U, pb, J0; // impulse approximation: binding / mc^2 of the shell, its momentum bound, J0
+    T qmax2; // Rayleigh, L >= 1: largest q^2
+    bool turns; // false: an impulse-approximation shell bound more tightly than E (no scatter)
+};
+
+// shell of an impulse-approximation Compton event (one draw;
+// comptonscatterIA, interactions.hpp)
+template <typename T>
+__device__ inline uint32_t impulseShell(const Tables<T>& t, uint32_t mat, T E, Rng& rng)
 {
     const Shell<T>* sh = t.shells + mat * 12;
     T cum[12];
     T acc = 0;
     for (int i = 0; i < 12; ++i) {
-        const T w = (sh[i].binding < p.energy && sh[i].j0 > 0) ? sh[i].electrons : T(0);
+        const T w = (sh[i].binding < E && sh[i].j0 > 0) ? sh[i].electrons : T(0);
         acc = i == 0 ? w : acc + w;
         cum[i] = acc;
     }
-    int si = 0;
+    uint32_t si = 0;
     const T sample = cum[11] * rng.uniform<T>();
     while (cum[si] < sample && si < 11)
         ++si;
-    const T U = sh[si].binding / T(ELECTRON_REST_MASS);
-    const T pb = sqrt(2 * U + U * U);
-    const T J0 = sh[si].j0;
-    const T k = p.energy / T(ELECTRON_REST_MASS);
-    if (U > k)
-        return 0;
-    const T emin = 1 / (1 + 2 * k);
-    const T gmaxInv = 1 / (1 / emin + emin);
-    T e, cosAngle;
-    bool rejected;
-    uint32_t trials = 0;
-    do {
-        const T r1 = rng.uniform<T>();
-        e = r1 + (1 - r1) * emin;
-        const T tt = (1 - e) / (k * e);
-        const T sin2 = tt * (2 - tt);
-        cosAngle = 1 - tt;
-        const T g = (1 / e + e - sin2) * gmaxInv;
-        const T r2 = rng.uniform<T>();
-        rejected = r2 > g;
-        if (!rejected) {
-            const T pi = (k * (k - U) * (1 - cosAngle) - U) / sqrt(2 * k * (k - U) * (1 - cosAngle) + U * U);
-            const T kc = k * e;
-            const T qc = sqrt(k * k + kc * kc - 2 * k * kc * cosAngle);
-            const T alpha = qc * (1 + kc * (kc - k * cosAngle) / (qc * qc)) / k;
-            const T bpart = 1 + 2 * J0 * fabs(pi);
-            const T b = (bpart * bpart + 1) / 2;
-            const T expb = exp(-b);
-            T S;
-            if (pi <= -pb) {
-                S = (1 - alpha * pb) * expb / 2;
-            } else if (pi < pb) {
-                constexpr T sp2 = 1 / (Consts<T>::INV_SQRTPI / Consts<T>::SQRT2);
-                const T pabs = fabs(pb);
-                const T piabs = fabs(pi);
-                const T part1 = alpha * sp2 / (4 * J0);
-                constexpr T a1 = T(0.34802);
-                constexpr T a2 = T(-0.0958798);
-                constexpr T a3 = T(0.7478556);
-                const T sqrte = sqrt(Consts<T>::E);
-                const T tp = 1 / (1 + T(0.332673) * (1 + 2 * J0 * pabs));
-                const T tpi = 1 / (1 + T(0.332673) * (1 + 2 * J0 * piabs));
-                const T part2p = sqrte - expb * tp * (a1 + a2 * tp + a3 * tp * tp);
-                const T part2pi = sqrte - expb * tpi * (a1 + a2 * tpi + a3 * tpi * tpi);
-                if (pi <= 0)
-                    S = (1 - alpha * pi) * expb / 2 - part1 * (part2p - part2pi);
-                else
-                    S = 1 - (1 - alpha * pi) * expb / 2 - part1 * (part2p - part2pi);
-            } else {
-                S = 1 - (1 - alpha * pb) * expb / 2;
-            }
-            const T r3 = rng.uniform<T>();
-            rejected = r3 > S;
-            if (!rejected) {
-                T Fmax;
-                if (pi <= -pb)
-                    Fmax = 1 - alpha * pb;
-                else if (pi >= pb)
-                    Fmax = 1 + alpha * pb;
-                else
-                    Fmax = 1 + alpha * pi;
-                const T r4 = rng.uniform<T>();
-                const T rbar2 = 2 * r4 * expb;
-                T pz;
-                if (rbar2 < 1) {
-                    const T part = sqrt(1 - 2 * log(rbar2));
-                    pz = (1 - part) / (2 * J0) / T(ELECTRON_REST_MASS);
-                } else {
-                    const T part = sqrt(1 - 2 * log(2 - rbar2));
-                    pz = (part - 1) / (2 * J0) / T(ELECTRON_REST_MASS);
-                }
-                T Fpz;
-                if (pz <= -pb)
-                    Fpz = 1 - alpha * pb;
-                else if (pz >= pb)
-                    Fpz = 1 + alpha * pb;
-                else
-                    Fpz = 1 + alpha * pz;
-                const T r5 = rng.uniform<T>();
-                rejected = r5 > Fpz / Fmax;
-                if (!rejected) {
-                    const T part = sqrt(1 - 2 * e * cosAngle + e * e * (1 - pz * pz * sin2));
-                    const T kbar = kc / (1 - pz * pz * e * e) * (1 - pz * pz * e * cosAngle + pz * part);
-                    e = kbar / k;
-                }
-            }
+    return si;
+}
+
+template <typename T, int L>
+__device__ inline ScatterSetup<T> comptonSetup(const Tables<T>& t, uint32_t mat, T E, uint32_t shell)
+{
+    ScatterSetup<T> s {};
+    s.turns = true;
+    if constexpr (L == 2) {
+        const Shell<T>& sh = t.shells[mat * 12 + shell];
+        s.U = sh.binding / T(ELECTRON_REST_MASS);
+        s.pb = sqrt(2 * s.U + s.U * s.U);
+        s.J0 = sh.j0;
+    }
+    s.k = E / T(ELECTRON_REST_MASS);
+    if constexpr (L == 2)
+        s.turns = !(s.U > s.k);
+    s.emin = 1 / (1 + 2 * s.k);
+    s.gmaxInv = 1 / (1 / s.emin + s.emin);
+    return s;
+}
+
+template <typename T, int L>
+__device__ inline ScatterSetup<T> rayleighSetup(T E)
+{
+    ScatterSetup<T> s {};
+    s.turns = true;
+    if constexpr (L >= 1) {
+        constexpr T kinv = T(1) / T(KEV_TO_ANGSTROM);
+        const T qmax = E * kinv;
+        s.qmax2 = qmax * qmax;
+    }
+    return s;
+}
+
+// One attempt of the Compton sampler: Klein-Nishina (L = 0), times the
+// scatter function (L = 1), the impulse approximation (L = 2). Sets the
+// kept energy fraction e and the cosine of the polar angle; true when
+// accepted.
+template <typename T, int L>
+__device__ inline bool comptonAttempt(const Tables<T>& t, uint32_t mat, T E, const ScatterSetup<T>& s, Rng& rng, T& e, T& cosAngle)
+{
+    const T r1 = rng.uniform<T>();
+    e = r1 + (1 - r1) * s.emin;
+    const T tt = (1 - e) / (s.k * e);
+    const T sin2 = tt * (2 - tt);
+    cosAngle = 1 - tt;
+    const T g = (1 / e + e - sin2) * s.gmaxInv;
+    const T r2 = rng.uniform<T>();
+    if constexpr (L == 0) {
+        return !(r2 > g);
+    } else if constexpr (L == 1) {
+        constexpr T kinv = T(1) / T(KEV_TO_ANGSTROM);
+        const T q = E * kinv * sqrt(T(0.5) - cosAngle * T(0.5));
+        return !(r2 > g * scatterFactor(t, mat, q));
+    } else {
+        if (r2 > g)
+            return false;
+        const T k = s.k, U = s.U, pb = s.pb, J0 = s.J0;
+        const T pi = (k * (k - U) * (1 - cosAngle) - U) / sqrt(2 * k * (k - U) * (1 - cosAngle) + U * U);
+        const T kc = k * e;
+        const T qc = sqrt(k * k + kc * kc - 2 * k * kc * cosAngle);
+        const T alpha = qc * (1 + kc * (kc - k * cosAngle) / (qc * qc)) / k;
+        const T bpart = 1 + 2 * J0 * fabs(pi);
+        const T b = (bpart * bpart + 1) / 2;
+        const T expb = exp(-b);
+        T S;
+        if (pi <= -pb) {
+            S = (1 - alpha * pb) * expb / 2;
+        } else if (pi < pb) {
+            constexpr T sp2 = 1 / (Consts<T>::INV_SQRTPI / Consts<T>::SQRT2);
+            const T pabs = fabs(pb);
+            const T piabs = fabs(pi);
+            const T part1 = alpha * sp2 / (4 * J0);
+            constexpr T a1 = T(0.34802);
+            constexpr T a2 = T(-0.0958798);
+            constexpr T a3 = T(0.7478556);
+            const T sqrte = sqrt(Consts<T>::E);
+            const T tp = 1 / (1 + T(0.332673) * (1 + 2 * J0 * pabs));
+            const T tpi = 1 / (1 + T(0.332673) * (1 + 2 * J0 * piabs));
+            const T part2p = sqrte - expb * tp * (a1 + a2 * tp + a3 * tp * tp);
+            const T part2pi = sqrte - expb * tpi * (a1 + a2 * tpi + a3 * tpi * tpi);
+            if (pi <= 0)
+                S = (1 - alpha * pi) * expb / 2 - part1 * (part2p - part2pi);
+            else
+                S = 1 - (1 - alpha * pi) * expb / 2 - part1 * (part2p - part2pi);
+        } else {
+            S = 1 - (1 - alpha * pb) * expb / 2;
         }
-    } while (rejected && ++trials < MAX_TRIALS);
-    const T theta = acos(cosAngle);
-    const T phi = rng.uniform<T>(Consts<T>::PI + Consts<T>::PI);
-    peturb(p.dir, theta, phi);
+        const T r3 = rng.uniform<T>();
+        if (r3 > S)
+            return false;
+        T Fmax;
+        if (pi <= -pb)
+            Fmax = 1 - alpha * pb;
+        else if (pi >= pb)
+            Fmax = 1 + alpha * pb;
+        else
+            Fmax = 1 + alpha * pi;
+        const T r4 = rng.uniform<T>();
+        const T rbar2 = 2 * r4 * expb;
+        T pz;
+        if (rbar2 < 1) {
+            const T part = sqrt(1 - 2 * log(rbar2));
+            pz = (1 - part) / (2 * J0) / T(ELECTRON_REST_MASS);
+        } else {
+            const T part = sqrt(1 - 2 * log(2 - rbar2));
+            pz = (part - 1) / (2 * J0) / T(ELECTRON_REST_MASS);
+        }
+        T Fpz;
+        if (pz <= -pb)
+            Fpz = 1 - alpha * pb;
+        else if (pz >= pb)
+            Fpz = 1 + alpha * pb;
+        else
+            Fpz = 1 + alpha * pz;
+        const T r5 = rng.uniform<T>();
+        if (r5 > Fpz / Fmax)
+            return false;
+        const T part = sqrt(1 - 2 * e * cosAngle + e * e * (1 - pz * pz * sin2));
+        const T kbar = kc / (1 - pz * pz * e * e) * (1 - pz * pz * e * cosAngle + pz * part);
+        e = kbar / k;
+        return true;
+    }
+}
+
+// One attempt of the Rayleigh sampler: L = 0 the polar angle itself from
+// (2 - sin^2) sin, L >= 1 the cosine from the form factor's RITA table and
+// the Thomson factor. 'angle' is the angle (L = 0) or its cosine.
+template <typename T, int L>
+__device__ inline bool rayleighAttempt(const Tables<T>& t, uint32_t mat, T E, const ScatterSetup<T>& s, Rng& rng, T& angle)
+{
+    if constexpr (L == 0) {
+        constexpr T extreme = (4 * Consts<T>::SQRT2) / (3 * Consts<T>::SQRT3);
+        const T r1 = rng.uniform<T>(T(0), extreme);
+        angle = rng.uniform<T>(T(0), Consts<T>::PI);
+        const T sn = sin(angle);
+        return !(r1 > ((2 - sn * sn) * sn));
+    } else {
+        const T q2 = ritaSample(t, mat, rng, s.qmax2);
+        const T invE = T(KEV_TO_ANGSTROM) / E;
+        angle = 1 - 2 * q2 * invE * invE;
+        return !((1 + angle * angle) * T(0.5) < rng.uniform<T>());
+    }
+}
+
+// The end of a Compton or Rayleigh event from its last attempt: the
+// azimuth draw and the turn of the direction (none for an impulse shell
+// that does not scatter), the kept energy fraction applied. Returns the
+// energy given to the medium (0 for Rayleigh).
+template <typename T, int L>
+__device__ inline T finishScatter(Particle<T>& p, Rng& rng, bool compton, bool turns, T e, T angle)
+{
+    if (turns) {
+        const Polar<T> polar = (L == 0 && !compton) ? polarOfAngle(angle) : polarOfCos(angle);
+        const T phi = rng.uniform<T>(Consts<T>::PI + Consts<T>::PI);
+        peturb(p.dir, polar, phi);
+    }
     const T E = p.energy;
-    p.energy *= e;
+    p.energy *= compton ? e : T(1);
     return E - p.energy;
+}
+
+// A Compton (compton = true) or Rayleigh event after the type draw, with
+// its rejection loop run to the end (at most MAX_TRIALS attempts).
+template <typename T, int L>
+__device__ inline T scatterEvent(const Tables<T>& t, Particle<T>& p, uint32_t mat, Rng& rng, bool compton)
+{
+    T e = 1, angle = 1;
+    bool turns = true;
+    if (compton) {
+        const uint32_t shell = L == 2 ? impulseShell(t, mat, p.energy, rng) : 0u;
+        const ScatterSetup<T> s = comptonSetup<T, L>(t, mat, p.energy, shell);
+        turns = s.turns;
+        uint32_t trials = 0;
+        if (turns)
+            while (!comptonAttempt<T, L>(t, mat, p.energy, s, rng, e, angle) && ++trials < MAX_TRIALS) { }
+    } else {
+        const ScatterSetup<T> s = rayleighSetup<T, L>(p.energy);
+        uint32_t trials = 0;
+        while (!rayleighAttempt<T, L>(t, mat, p.energy, s, rng, angle) && ++trials < MAX_TRIALS) { }
+    }
+    return finishScatter<T, L>(p, rng, compton, turns, e, angle);
 }
 
 template <typename T, int L>
 __device__ inline T comptonScatter(const Tables<T>& t, Particle<T>& p, uint32_t mat, Rng& rng)
 {
-    if constexpr (L == 2) {
-        return comptonImpulse(t, p, mat, rng);
-    } else {
-        const T E = p.energy;
-        const T k = E / T(ELECTRON_REST_MASS);
-        const T emin = 1 / (1 + 2 * k);
-        const T gmaxInv = 1 / (1 / emin + emin);
-        T e, cosAngle;
-        bool rejected;
-        uint32_t trials = 0;
-        do {
-            const T r1 = rng.uniform<T>();
-            e = r1 + (1 - r1) * emin;
-            const T tt = (1 - e) / (k * e);
-            const T sin2 = tt * (2 - tt);
-            cosAngle = 1 - tt;
-            const T g = (1 / e + e - sin2) * gmaxInv;
-            const T r2 = rng.uniform<T>();
-            if constexpr (L == 1) {
-                constexpr T kinv = T(1) / T(KEV_TO_ANGSTROM);
-                const T q = E * kinv * sqrt(T(0.5) - cosAngle * T(0.5));
-                rejected = r2 > g * scatterFactor(t, mat, q);
-            } else {
-                rejected = r2 > g;
-            }
-        } while (rejected && ++trials < MAX_TRIALS);
-        const T theta = acos(cosAngle);
-        const T phi = rng.uniform<T>(Consts<T>::PI + Consts<T>::PI);
-        peturb(p.dir, theta, phi);
-        p.energy *= e;
-        return E - p.energy;
-    }
+    return scatterEvent<T, L>(t, p, mat, rng, true);
+}
+
+template <typename T, int L>
+__device__ inline void rayleighScatter(const Tables<T>& t, Particle<T>& p, uint32_t mat, Rng& rng)
+{
+    scatterEvent<T, L>(t, p, mat, rng, false);
 }
 
 // Energy imparted by an event, with the 1 keV cutoff: below it the photon is
@@ -765,15 +836,12 @@ __device__ inline bool computeInteractions(const Tables<T>& t, const T att[3], P
 {
     const T total = (att[0] + att[1]) + att[2];
     const T r3 = rng.uniform<T>(total);
-    if (r3 < att[0]) {
-        const T e = photoAbsorption<T, L>(t, p, mat, rng);
-        return depositAfterEvent(sc, idx, p, e, updateMajorant);
-    } else if (r3 < (att[0] + att[1])) {
-        const T e = comptonScatter<T, L>(t, p, mat, rng);
-        return depositAfterEvent(sc, idx, p, e, updateMajorant);
-    }
-    rayleighScatter<T, L>(t, p, mat, rng);
-    return true;
+    // one scoring call site for photoabsorption and Compton (a wave whose
+    // lanes hold both runs the atomics once)
+    const bool photo = r3 < att[0];
+    const bool compton = !photo && r3 < (att[0] + att[1]);
+    const T e = photo ? photoAbsorption<T, L>(t, p, mat, rng) : scatterEvent<T, L>(t, p, mat, rng, compton);
+    return (photo || compton) ? depositAfterEvent(sc, idx, p, e, updateMajorant) : true;
 }
 
 template <typename T, int L>
@@ -796,13 +864,10 @@ __device__ inline bool computeInteractionsForced(const Tables<T>& t, T eventProb
     const T r1 = rng.uniform<T>();
     if (r1 < eventProbability * (1 - photoProbability)) {
         const T r2 = rng.uniform<T>(att[1] + att[2]);
-        if (r2 < att[1]) {
-            const T e = comptonScatter<T, L>(t, p, mat, rng);
-            if (!depositAfterEvent(sc, idx, p, e, updateMajorant))
-                return false;
-        } else {
-            rayleighScatter<T, L>(t, p, mat, rng);
-        }
+        const bool compton = r2 < att[1];
+        const T e = scatterEvent<T, L>(t, p, mat, rng, compton);
+        if (compton && !depositAfterEvent(sc, idx, p, e, updateMajorant))
+            return false;
     }
     p.weight *= (1 - weightCorrection);
     return true;
@@ -830,7 +895,23 @@ __device__ inline void readVoxel<float>(const World<float>& w, uint64_t idx, uin
         v = static_cast<const uint2*>(w.voxels)[idx];
     density = __uint_as_float(v.x);
     mat = v.y & 0xffu;
-    meas = (v.y >> 8) & 0xffu;
+    meas = v.y >> 24;
+}
+
+// The FP32 record's material and measurement flag as they are stored,
+// material | measurement << 24 (the FP32 kernel keeps them in one register:
+// 24-bit products with the material, as the LUT row index, ignore the flag)
+__device__ inline void readVoxelPacked(const World<float>& w, uint64_t idx, uint32_t& matMeas, float& density)
+{
+    uint2 v;
+    if (w.voxelBytes == 1)
+        v = w.palette[static_cast<const uint8_t*>(w.voxels)[idx]];
+    else if (w.voxelBytes == 2)
+        v = w.palette[static_cast<const uint16_t*>(w.voxels)[idx]];
+    else
+        v = static_cast<const uint2*>(w.voxels)[idx];
+    density = __uint_as_float(v.x);
+    matMeas = v.y;
 }
 
 // One Woodcock delta-tracking step (body of the loop in transport.hpp:650-699):

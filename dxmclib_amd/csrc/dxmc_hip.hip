@@ -838,6 +838,9 @@ int runLaunches(dxmc_ctx* c, const dxmc_world* w, const dxmc_tables* t, uint64_t
     uint64_t chunk = cfg.histories_per_launch;
     if (chunk == 0)
         chunk = cfg.precision == DXMC_PRECISION_FP64 ? (uint64_t(1) << 22) : (uint64_t(1) << 27);
+    // the FP32 kernel keeps a history's offset in its launch in 32 bits
+    if (cfg.precision != DXMC_PRECISION_FP64)
+        chunk = std::min<uint64_t>(chunk, uint64_t(1) << 32);
     const uint32_t block = cfg.block_size ? cfg.block_size : 256;
     if (block % 64 != 0 || block > 256)
         return c->fail(DXMC_ERR_ARGUMENT, "run: block size must be a multiple of 64 up to 256");

@@ -38,6 +38,15 @@ using namespace dxmc_dev;
 namespace {
 
 enum : uint32_t { DEAD = 0, FLYING = 1, PENDING = 2 };
+enum : uint32_t { EVENT_NEW = 0, EVENT_COMPTON = 1, EVENT_RAYLEIGH = 2 };
+
+// Rejection-loop attempts of a Compton or Rayleigh event per interaction
+// pass. A pass runs each sampler until its slowest lane accepts, so an
+// uncapped pass costs the longest of its lanes' loops; with a cap, a lane
+// that has not accepted stays pending with its attempt count and continues
+// in a later pass with the following draws of its stream (the same draws
+// in the same order as the uncapped loop).
+constexpr uint32_t SCATTER_ATTEMPTS = 2;
 
 __device__ inline uint32_t rankBelow(uint64_t mask)
 {
@@ -149,8 +158,11 @@ __device__ inline void stageTables(Tables<float>& t, unsigned char* lds)
 constexpr float LN2F = 0.693147180559945309f;
 
 // per-wave queue of sampled photons in LDS: position, direction, energy,
-// weight, PCG state and increment (the LUT point and majorant follow from
-// the energy when a lane takes the photon)
+// weight, PCG state, the history's offset in the launch (its PCG increment
+// is 2 (first history + launch begin + offset) + 1) and the LUT segment of
+// the energy, found while the whole wave samples (the segment search has
+// loops; a lane that takes the photon only adds the segment offset and the
+// majorant)
 constexpr int QUEUE_FIELDS = 12;
 constexpr uint32_t QUEUE_CAP = 64;
 constexpr uint32_t QUEUE_BYTES = QUEUE_FIELDS * QUEUE_CAP * 4;
@@ -197,15 +209,24 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(W, 8))) voi
     uint32_t poolExp = 0;
     bool claimedAll = span == 0;
     Scorer sc { tally, 0 };
+    // in-world steps of the lane's finished histories, and of its current one
+    // (added to 'steps' when the lane takes its next photon or at the end)
     uint32_t steps = 0;
     uint32_t historySteps = 0;
     // the pending interaction
     uint64_t idx = 0;
-    uint32_t mat = 0, meas = 0;
+    // material | measurement flag << 24 of the voxel reached (one register;
+    // the LUT row products are 24-bit and ignore the flag)
+    uint32_t matMeas = 0;
     float density = 0;
+    // the pending event's stage: bits 0-1 new / Compton / Rayleigh, bits 2-5
+    // the impulse-approximation shell, bits 8-17 attempts made
+    uint32_t istate = EVENT_NEW;
     // the photon's place in the LUT; changes only with its energy
     LutPoint<float> lp { 0, 0.0f };
     const float inv[3] = { 1.0f / w.spacing[0], 1.0f / w.spacing[1], 1.0f / w.spacing[2] };
+    // global index of the launch's first history (PCG increments of queued photons)
+    const uint64_t launchFirst = firstHistory + hBegin;
     const uint32_t lane = __lane_id();
 
     // the wave's queue of new photons, already in the world (LDS, SoA)
@@ -226,6 +247,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(W, 8))) voi
             bool sampled = false;
             Particle<float> np;
             Rng nr { 0, 1 };
+            uint32_t nOffset = 0;
             while (assigned < want) {
                 if (poolNext >= poolEnd) {
                     if (claimedAll)
@@ -255,6 +277,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(W, 8))) voi
                     while (h >= expStart[e + 1])
                         ++e;
                     nr = Rng::forHistory(seed, firstHistory + h);
+                    nOffset = static_cast<uint32_t>(h - hBegin);
                     sampleParticle(exps[e], t, nr, np);
                     sampled = transportToWorld(w, np);
                 }
@@ -265,8 +288,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(W, 8))) voi
             if (sampled) {
                 const uint32_t slot = queued + rankBelow(entered);
                 const float v[QUEUE_FIELDS] = { np.pos[0], np.pos[1], np.pos[2], np.dir[0], np.dir[1], np.dir[2], np.energy, np.weight,
-                    __uint_as_float(static_cast<uint32_t>(nr.s)), __uint_as_float(static_cast<uint32_t>(nr.s >> 32)),
-                    __uint_as_float(static_cast<uint32_t>(nr.inc)), __uint_as_float(static_cast<uint32_t>(nr.inc >> 32)) };
+                    __uint_as_float(static_cast<uint32_t>(nr.s)), __uint_as_float(static_cast<uint32_t>(nr.s >> 32)), __uint_as_float(nOffset),
+                    __uint_as_float(lutPoint(t, np.energy).k) };
                 for (int f = 0; f < QUEUE_FIELDS; ++f)
                     q[f * QUEUE_CAP + slot] = v[f];
             }
@@ -289,10 +312,13 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(W, 8))) voi
                     p.energy = v[6];
                     p.weight = v[7];
                     rng.s = static_cast<uint64_t>(__float_as_uint(v[9])) << 32 | __float_as_uint(v[8]);
-                    rng.inc = static_cast<uint64_t>(__float_as_uint(v[11])) << 32 | __float_as_uint(v[10]);
-                    lp = lutPoint(t, p.energy);
+                    rng.inc = 2 * (launchFirst + __float_as_uint(v[10])) + 1;
+                    // the LUT point lutPoint() gives, from the queued segment
+                    lp.k = __float_as_uint(v[11]);
+                    lp.x = __builtin_amdgcn_logf(p.energy) - t.lutAnchor[lp.k];
                     majInv = majorantInverseAt(t, lp);
                     update = false;
+                    steps += historySteps;
                     historySteps = 0;
                     phase = FLYING;
                 }
@@ -309,19 +335,71 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(W, 8))) voi
         const uint32_t nPend = __popcll(pend);
         if (nPend != 0 && (nPend >= interactThreshold || fly == 0)) {
             if (phase == PENDING) {
-                float att[3];
-                attenuationAt(t, mat, lp, att);
-                const float prob = ((att[0] + att[1]) + att[2]) * density * majInv;
-                bool alive = meas == 0 ? computeInteractions<float, L>(t, att, p, mat, sc, idx, rng, update)
-                                       : computeInteractionsForced<float, L>(t, prob, att, p, mat, sc, idx, rng, update);
-                if (alive)
-                    alive = roulette(p, rng);
-                if (alive && update) { // new energy: its LUT segment and majorant
-                    lp = lutPoint(t, p.energy);
-                    majInv = majorantInverseAt(t, lp);
-                    update = false;
+                // computeInteractions with the Compton and Rayleigh rejection
+                // loops cut into at most SCATTER_ATTEMPTS attempts per pass
+                // done: the event ends in this pass (a new photoabsorption or
+                // forced event, or a Compton/Rayleigh event whose attempt is
+                // accepted); otherwise the lane stays pending
+                bool done = false, alive = true, scores = false;
+                float deposit = 0.0f;
+                const uint32_t mat = matMeas & 0xffu;
+                if ((istate & 3u) == EVENT_NEW) {
+                    float att[3];
+                    attenuationAt(t, mat, lp, att);
+                    if (matMeas > 0xffffffu) { // forced event (measurement voxel): whole in one pass
+                        const float prob = ((att[0] + att[1]) + att[2]) * density * majInv;
+                        alive = computeInteractionsForced<float, L>(t, prob, att, p, mat, sc, idx, rng, update);
+                        done = true;
+                    } else {
+                        const float r3 = rng.uniform<float>((att[0] + att[1]) + att[2]);
+                        if (r3 < att[0]) {
+                            deposit = photoAbsorption<float, L>(t, p, mat, rng);
+                            scores = true;
+                            done = true;
+                        } else {
+                            const bool compton = r3 < (att[0] + att[1]);
+                            const uint32_t shell = L == 2 && compton ? impulseShell(t, mat, p.energy, rng) : 0u;
+                            istate = (compton ? EVENT_COMPTON : EVENT_RAYLEIGH) | shell << 2;
+                        }
+                    }
                 }
-                phase = alive ? FLYING : DEAD;
+                if (!done) { // a Compton or Rayleigh event, new or continued
+                    const bool compton = (istate & 3u) == EVENT_COMPTON;
+                    uint32_t attempts = istate >> 8;
+                    float e = 1.0f, angle = 1.0f;
+                    bool finished = false, turns = true;
+                    if (compton) {
+                        const ScatterSetup<float> su = comptonSetup<float, L>(t, mat, p.energy, (istate >> 2) & 15u);
+                        turns = su.turns;
+                        finished = !turns;
+                        for (uint32_t a = 0; a < SCATTER_ATTEMPTS && !finished; ++a)
+                            finished = comptonAttempt<float, L>(t, mat, p.energy, su, rng, e, angle) || ++attempts >= MAX_TRIALS;
+                    } else {
+                        const ScatterSetup<float> su = rayleighSetup<float, L>(p.energy);
+                        for (uint32_t a = 0; a < SCATTER_ATTEMPTS && !finished; ++a)
+                            finished = rayleighAttempt<float, L>(t, mat, p.energy, su, rng, angle) || ++attempts >= MAX_TRIALS;
+                    }
+                    if (finished) {
+                        deposit = finishScatter<float, L>(p, rng, compton, turns, e, angle);
+                        scores = compton;
+                        done = true;
+                    } else {
+                        istate = (istate & 0xffu) | attempts << 8; // stays pending
+                    }
+                }
+                if (done) {
+                    if (scores)
+                        alive = depositAfterEvent(sc, idx, p, deposit, update);
+                    if (alive)
+                        alive = roulette(p, rng);
+                    if (alive && update) { // new energy: its LUT segment and majorant
+                        lp = lutPoint(t, p.energy);
+                        majInv = majorantInverseAt(t, lp);
+                        update = false;
+                    }
+                    istate = EVENT_NEW;
+                    phase = alive ? FLYING : DEAD;
+                }
             }
         } else {
             // Woodcock flights of the FLYING lanes, one after the other
@@ -356,20 +434,19 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(W, 8))) voi
                     if (w.codeBytes != 0) {
                         const uint32_t code = w.codeBytes == 1 ? static_cast<const uint8_t*>(w.codes)[idx] : static_cast<const uint16_t*>(w.codes)[idx];
                         const DensityBound b = w.bounds[code];
-                        mat = b.material;
-                        meas = 0;
+                        matMeas = b.material; // code worlds carry no measurement flags
                         dLo = b.lo;
                         dHi = b.hi;
                         density = b.lo;
                     } else {
-                        readVoxel(w, idx, mat, density, meas);
+                        readVoxelPacked(w, idx, matMeas, density);
                         dLo = density;
                         dHi = density;
                     }
                     bool settled = false;
                     prob = 0.0f;
                     if (t.lutSumBound) {
-                        const float2 sb = t.lutSumBound[__umul24(mat, t.lutNodes) + lp.k];
+                        const float2 sb = t.lutSumBound[__umul24(matMeas, t.lutNodes) + lp.k];
                         const float lo = sb.x * dLo * majInv;
                         const float hi = sb.y * dHi * majInv;
                         settled = (u2 < lo) | (u2 >= hi);
@@ -379,12 +456,13 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(W, 8))) voi
                         if (w.codeBytes != 0)
                             density = __uint_as_float(static_cast<const uint2*>(w.voxels)[idx].x);
                         float att[3];
-                        attenuationAt(t, mat, lp, att);
+                        attenuationAt(t, matMeas, lp, att); // 24-bit row product: flag ignored
                         prob = ((att[0] + att[1]) + att[2]) * density * majInv;
                     }
-                    const bool drawn = in & (meas == 0);
+                    const bool measured = matMeas > 0xffffffu;
+                    const bool drawn = in & !measured;
                     rng.s = drawn ? r2.s : rng.s;
-                    const bool real = (meas != 0) | (u2 < prob);
+                    const bool real = measured | (u2 < prob);
                     uint32_t next = in ? (real ? PENDING : FLYING) : DEAD;
                     if (next == FLYING && p.energy * p.weight < 5.0f) { // Russian roulette
                         if (rng.uniform<float>() < 0.8f)
@@ -392,7 +470,6 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(W, 8))) voi
                         else
                             p.weight *= 1.0f / (1.0f - 0.8f);
                     }
-                    steps += in ? 1u : 0u;
                     historySteps += in ? 1u : 0u;
                     phase = (next != DEAD && historySteps >= MAX_STEPS) ? DEAD : next;
                 }
@@ -406,7 +483,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(W, 8))) voi
         }
     }
     if (ctr)
-        dxmc_backend::accumulateCounters(ctr, steps, sc.events);
+        dxmc_backend::accumulateCounters(ctr, steps + historySteps, sc.events);
 }
 
 }

@@ -9,7 +9,7 @@
 //                   :52-58), whether any measurement flag is set, and the
 //                   FP64 density copy the normalisation reads
 //   paletteInsert   distinct FP32 voxel records (density bits, material |
-//                   measurement << 8) into an open-addressing hash table
+//                   measurement << 24) into an open-addressing hash table
 //   paletteCompact  palette index of each distinct record
 //   paletteIndex<I> 1- or 2-byte palette index of every voxel
 //   packVoxels      8-byte records when there are more than 65536
@@ -40,7 +40,7 @@ constexpr uint32_t HASH_BITS = 20;
 constexpr uint32_t HASH_SLOTS = 1u << HASH_BITS;
 constexpr uint32_t HASH_PROBES = 256; // longer probe sequences fall back to 8-byte records
 constexpr uint32_t COMPACT_BLOCK = 1024;
-constexpr unsigned long long EMPTY = ~0ull; // not a record key: the tag is < 2^16
+constexpr unsigned long long EMPTY = ~0ull; // not a record key: a tag is at most 0xff0000ff
 
 uint32_t gridFor(uint64_t n, uint32_t block, uint32_t cap) { return static_cast<uint32_t>(std::min<uint64_t>((n + block - 1) / block, cap)); }
 
@@ -62,7 +62,7 @@ double fromOrderKey(unsigned long long k)
 
 __device__ inline unsigned long long recordKey(const uint8_t* mat, const double* dens, const uint8_t* meas, uint64_t i)
 {
-    const uint32_t tag = static_cast<uint32_t>(mat[i]) | (meas ? static_cast<uint32_t>(meas[i]) << 8 : 0u);
+    const uint32_t tag = static_cast<uint32_t>(mat[i]) | (meas ? static_cast<uint32_t>(meas[i]) << 24 : 0u);
     return static_cast<unsigned long long>(tag) << 32 | __float_as_uint(static_cast<float>(dens[i]));
 }
 
@@ -234,7 +234,7 @@ __global__ __launch_bounds__(SCAN_BLOCK) void packVoxels(const uint8_t* __restri
     for (uint64_t i = static_cast<uint64_t>(blockIdx.x) * SCAN_BLOCK + threadIdx.x; i < n; i += stride) {
         uint2 v;
         v.x = __float_as_uint(static_cast<float>(dens[i]));
-        v.y = static_cast<uint32_t>(mat[i]) | (meas ? static_cast<uint32_t>(meas[i]) << 8 : 0u);
+        v.y = static_cast<uint32_t>(mat[i]) | (meas ? static_cast<uint32_t>(meas[i]) << 24 : 0u);
         vox[i] = v;
     }
 }

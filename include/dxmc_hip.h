@@ -209,7 +209,7 @@ typedef struct {
     int32_t low_energy_correction; /* 0 NONE, 1 LIVERMORE, 2 IA (lowenergycorrectionmodel.hpp) */
     uint64_t seed; /* history h uses state1 = 2h+1, state0 = splitmix64(seed ^ splitmix64(h)) */
     uint64_t first_history; /* global index of the first history of exposure 0 */
-    uint64_t histories_per_launch; /* 0: automatic; bounds cancel/progress latency */
+    uint64_t histories_per_launch; /* 0: automatic; bounds cancel/progress latency; FP32 at most 2^32 */
     uint32_t block_size; /* 0: tuned default */
     uint32_t blocks_per_cu; /* FP32 persistent kernel occupancy, 0: tuned default */
     uint32_t regeneration_threshold; /* FP32: dead lanes per wave before they take queued new photons, 0: tuned */

@@ -283,20 +283,25 @@ def test_transport_water_pencil_fp64(device, L):
     assert abs(stats.events - ref["events"]) <= 0.01 * ref["events"]
 
 
-def test_transport_water_pencil_fp32_vs_fp64(device):
+@pytest.mark.parametrize("L", [0, 1, 2])
+def test_transport_water_pencil_fp32_vs_fp64(device, L):
+    print(f"water pencil fp32 vs fp64 L={L}", flush=True)
     wl = workloads.water_pencil()
-    d64, _, v64, s64, *_ = _run_device(device, wl, 1, capi.PRECISION_FP64)
-    d32, _, v32, s32, *_ = _run_device(device, wl, 1, capi.PRECISION_FP32)
+    d64, _, v64, s64, *_ = _run_device(device, wl, L, capi.PRECISION_FP64)
+    d32, _, v32, s32, *_ = _run_device(device, wl, L, capi.PRECISION_FP32)
     rel = abs(d32.sum() - d64.sum()) / d64.sum()
     frac, n = _zstats(d32, v32, d64, v64)
     hot = d64 > 0.1 * d64.max()
     err = np.abs(d32[hot] - d64[hot]) / d64[hot]
-    print(f"FP32 vs FP64: total {rel:.2e}, |z|<3 {frac:.5f} of {n}, voxels >10% max: mean err {err.mean():.2e} max {err.max():.2e}")
-    assert rel < 1e-3
+    print(f"FP32 vs FP64: total {rel:.2e}, |z|<3 {frac:.5f} of {n}, voxels >10% max: mean err {err.mean():.2e} max {err.max():.2e}, "
+          f"steps {s32.steps} / {s64.steps}, events {s32.events} / {s64.events}")
+    assert rel < 1e-5
     assert frac >= 0.997
-    # identical streams: same histories, so counts agree closely
-    assert abs(s32.steps - s64.steps) <= 1e-3 * s64.steps
-    assert abs(s32.events - s64.events) <= 1e-3 * s64.events
+    # identical streams: the same histories, apart from the few whose path
+    # an FP32 rounding changes (1-3 of 1.45e6 steps and 1-2 of 1.27e6
+    # events): a sampler that loses or repeats draws moves these by percents
+    assert abs(s32.steps - s64.steps) <= 2e-5 * s64.steps
+    assert abs(s32.events - s64.events) <= 2e-5 * s64.events
     # the accuracy target: below 0.1 % in voxels above 10 % of the maximum
     assert err.max() < 1e-3
 
