@@ -588,9 +588,8 @@ __device__ inline T photoAbsorption(const Tables<T>& t, Particle<T>& p, uint32_t
 // photon only, and for the impulse approximation the shell drawn first) and
 // one attempt of their rejection loops (its draws and the acceptance test).
 // scatterEvent() below runs attempts until one is accepted or MAX_TRIALS are
-// spent, as the reference's loops do; the FP32 kernel runs a bounded number
-// of attempts per interaction pass and keeps the count between passes,
-// which draws the same numbers in the same order.
+// spent, as the reference's loops do, and ends both event types with one
+// copy of the azimuth draw and the turn of the direction.
 template <typename T>
 struct ScatterSetup {
     T k, emin, gmaxInv; // Compton: E / mc^2, smallest kept energy fraction, 1 / largest KN weight
@@ -901,17 +900,20 @@ __device__ inline void readVoxel<float>(const World<float>& w, uint64_t idx, uin
 // The FP32 record's material and measurement flag as they are stored,
 // material | measurement << 24 (the FP32 kernel keeps them in one register:
 // 24-bit products with the material, as the LUT row index, ignore the flag)
+// (the record is read as one 8-byte integer: read as a uint2 whose halves
+// are used apart, the compiler issues two 4-byte loads)
 __device__ inline void readVoxelPacked(const World<float>& w, uint64_t idx, uint32_t& matMeas, float& density)
 {
-    uint2 v;
+    const uint64_t* rec = reinterpret_cast<const uint64_t*>(w.palette);
+    uint64_t v;
     if (w.voxelBytes == 1)
-        v = w.palette[static_cast<const uint8_t*>(w.voxels)[idx]];
+        v = rec[static_cast<const uint8_t*>(w.voxels)[idx]];
     else if (w.voxelBytes == 2)
-        v = w.palette[static_cast<const uint16_t*>(w.voxels)[idx]];
+        v = rec[static_cast<const uint16_t*>(w.voxels)[idx]];
     else
-        v = static_cast<const uint2*>(w.voxels)[idx];
-    density = __uint_as_float(v.x);
-    matMeas = v.y;
+        v = static_cast<const uint64_t*>(w.voxels)[idx];
+    density = __uint_as_float(static_cast<uint32_t>(v));
+    matMeas = static_cast<uint32_t>(v >> 32);
 }
 
 // One Woodcock delta-tracking step (body of the loop in transport.hpp:650-699):

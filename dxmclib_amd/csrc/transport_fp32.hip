@@ -38,15 +38,7 @@ using namespace dxmc_dev;
 namespace {
 
 enum : uint32_t { DEAD = 0, FLYING = 1, PENDING = 2 };
-enum : uint32_t { EVENT_NEW = 0, EVENT_COMPTON = 1, EVENT_RAYLEIGH = 2 };
 
-// Rejection-loop attempts of a Compton or Rayleigh event per interaction
-// pass. A pass runs each sampler until its slowest lane accepts, so an
-// uncapped pass costs the longest of its lanes' loops; with a cap, a lane
-// that has not accepted stays pending with its attempt count and continues
-// in a later pass with the following draws of its stream (the same draws
-// in the same order as the uncapped loop).
-constexpr uint32_t SCATTER_ATTEMPTS = 2;
 
 __device__ inline uint32_t rankBelow(uint64_t mask)
 {
@@ -219,9 +211,6 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(W, 8))) voi
     // the LUT row products are 24-bit and ignore the flag)
     uint32_t matMeas = 0;
     float density = 0;
-    // the pending event's stage: bits 0-1 new / Compton / Rayleigh, bits 2-5
-    // the impulse-approximation shell, bits 8-17 attempts made
-    uint32_t istate = EVENT_NEW;
     // the photon's place in the LUT; changes only with its energy
     LutPoint<float> lp { 0, 0.0f };
     const float inv[3] = { 1.0f / w.spacing[0], 1.0f / w.spacing[1], 1.0f / w.spacing[2] };
@@ -335,71 +324,24 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(W, 8))) voi
         const uint32_t nPend = __popcll(pend);
         if (nPend != 0 && (nPend >= interactThreshold || fly == 0)) {
             if (phase == PENDING) {
-                // computeInteractions with the Compton and Rayleigh rejection
-                // loops cut into at most SCATTER_ATTEMPTS attempts per pass
-                // done: the event ends in this pass (a new photoabsorption or
-                // forced event, or a Compton/Rayleigh event whose attempt is
-                // accepted); otherwise the lane stays pending
-                bool done = false, alive = true, scores = false;
-                float deposit = 0.0f;
                 const uint32_t mat = matMeas & 0xffu;
-                if ((istate & 3u) == EVENT_NEW) {
-                    float att[3];
-                    attenuationAt(t, mat, lp, att);
-                    if (matMeas > 0xffffffu) { // forced event (measurement voxel): whole in one pass
-                        const float prob = ((att[0] + att[1]) + att[2]) * density * majInv;
-                        alive = computeInteractionsForced<float, L>(t, prob, att, p, mat, sc, idx, rng, update);
-                        done = true;
-                    } else {
-                        const float r3 = rng.uniform<float>((att[0] + att[1]) + att[2]);
-                        if (r3 < att[0]) {
-                            deposit = photoAbsorption<float, L>(t, p, mat, rng);
-                            scores = true;
-                            done = true;
-                        } else {
-                            const bool compton = r3 < (att[0] + att[1]);
-                            const uint32_t shell = L == 2 && compton ? impulseShell(t, mat, p.energy, rng) : 0u;
-                            istate = (compton ? EVENT_COMPTON : EVENT_RAYLEIGH) | shell << 2;
-                        }
-                    }
+                float att[3];
+                attenuationAt(t, mat, lp, att);
+                bool alive;
+                if (matMeas > 0xffffffu) { // forced event (measurement voxel)
+                    const float prob = ((att[0] + att[1]) + att[2]) * density * majInv;
+                    alive = computeInteractionsForced<float, L>(t, prob, att, p, mat, sc, idx, rng, update);
+                } else {
+                    alive = computeInteractions<float, L>(t, att, p, mat, sc, idx, rng, update);
                 }
-                if (!done) { // a Compton or Rayleigh event, new or continued
-                    const bool compton = (istate & 3u) == EVENT_COMPTON;
-                    uint32_t attempts = istate >> 8;
-                    float e = 1.0f, angle = 1.0f;
-                    bool finished = false, turns = true;
-                    if (compton) {
-                        const ScatterSetup<float> su = comptonSetup<float, L>(t, mat, p.energy, (istate >> 2) & 15u);
-                        turns = su.turns;
-                        finished = !turns;
-                        for (uint32_t a = 0; a < SCATTER_ATTEMPTS && !finished; ++a)
-                            finished = comptonAttempt<float, L>(t, mat, p.energy, su, rng, e, angle) || ++attempts >= MAX_TRIALS;
-                    } else {
-                        const ScatterSetup<float> su = rayleighSetup<float, L>(p.energy);
-                        for (uint32_t a = 0; a < SCATTER_ATTEMPTS && !finished; ++a)
-                            finished = rayleighAttempt<float, L>(t, mat, p.energy, su, rng, angle) || ++attempts >= MAX_TRIALS;
-                    }
-                    if (finished) {
-                        deposit = finishScatter<float, L>(p, rng, compton, turns, e, angle);
-                        scores = compton;
-                        done = true;
-                    } else {
-                        istate = (istate & 0xffu) | attempts << 8; // stays pending
-                    }
+                if (alive)
+                    alive = roulette(p, rng);
+                if (alive && update) { // new energy: its LUT segment and majorant
+                    lp = lutPoint(t, p.energy);
+                    majInv = majorantInverseAt(t, lp);
+                    update = false;
                 }
-                if (done) {
-                    if (scores)
-                        alive = depositAfterEvent(sc, idx, p, deposit, update);
-                    if (alive)
-                        alive = roulette(p, rng);
-                    if (alive && update) { // new energy: its LUT segment and majorant
-                        lp = lutPoint(t, p.energy);
-                        majInv = majorantInverseAt(t, lp);
-                        update = false;
-                    }
-                    istate = EVENT_NEW;
-                    phase = alive ? FLYING : DEAD;
-                }
+                phase = alive ? FLYING : DEAD;
             }
         } else {
             // Woodcock flights of the FLYING lanes, one after the other
@@ -433,11 +375,13 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(W, 8))) voi
                     float dLo, dHi;
                     if (w.codeBytes != 0) {
                         const uint32_t code = w.codeBytes == 1 ? static_cast<const uint8_t*>(w.codes)[idx] : static_cast<const uint16_t*>(w.codes)[idx];
-                        const DensityBound b = w.bounds[code];
-                        matMeas = b.material; // code worlds carry no measurement flags
-                        dLo = b.lo;
-                        dHi = b.hi;
-                        density = b.lo;
+                        // the interval as one 16-byte load (read field by field,
+                        // the compiler splits it into two loads on the chain)
+                        const uint4 b = *reinterpret_cast<const uint4*>(w.bounds + code);
+                        matMeas = b.z; // material; code worlds carry no measurement flags
+                        dLo = __uint_as_float(b.x);
+                        dHi = __uint_as_float(b.y);
+                        density = dLo;
                     } else {
                         readVoxelPacked(w, idx, matMeas, density);
                         dLo = density;
