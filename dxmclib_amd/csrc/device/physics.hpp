@@ -902,13 +902,13 @@ __device__ inline void readVoxel<float>(const World<float>& w, uint64_t idx, uin
 // 24-bit products with the material, as the LUT row index, ignore the flag)
 // (the record is read as one 8-byte integer: read as a uint2 whose halves
 // are used apart, the compiler issues two 4-byte loads)
-__device__ inline void readVoxelPacked(const World<float>& w, uint64_t idx, uint32_t& matMeas, float& density)
+__device__ inline void readVoxelPacked(const World<float>& w, uint32_t voxelBytes, uint64_t idx, uint32_t& matMeas, float& density)
 {
     const uint64_t* rec = reinterpret_cast<const uint64_t*>(w.palette);
     uint64_t v;
-    if (w.voxelBytes == 1)
+    if (voxelBytes == 1)
         v = rec[static_cast<const uint8_t*>(w.voxels)[idx]];
-    else if (w.voxelBytes == 2)
+    else if (voxelBytes == 2)
         v = rec[static_cast<const uint16_t*>(w.voxels)[idx]];
     else
         v = static_cast<const uint64_t*>(w.voxels)[idx];

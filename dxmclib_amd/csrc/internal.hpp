@@ -367,7 +367,8 @@ inline std::tuple<const void*, const void*, int, int> tuneKey(const dxmc_world* 
 int autotune(dxmc_ctx* c, const dxmc_world* w, const dxmc_tables* t, const dxmc_exposure* x, uint64_t nExp, const dxmc_run_cfg& base,
     uint64_t trialHistories, dxmc_run_cfg& best, dxmc_tune_report* report);
 // resident blocks per CU of the FP32 kernel (transport_fp32.hip)
-uint32_t residentBlocksFp32(int L, uint32_t wavesPerSimd, uint32_t ldsPerBlock);
+// resident blocks per CU of the FP32 kernel for a world kind (0: any world)
+uint32_t residentBlocksFp32(int L, uint32_t wavesPerSimd, uint32_t ldsPerBlock, uint32_t worldKind = 0);
 int runTransport(dxmc_ctx* c, const dxmc_world* w, const dxmc_tables* t, const dxmc_exposure* x, uint64_t nExp, const dxmc_run_cfg& cfg,
     dxmc_tally* tally, const dxmc_run_monitor* monitor, dxmc_run_stats* stats);
 // device inputs of a run's launches: exposure records of both precisions,

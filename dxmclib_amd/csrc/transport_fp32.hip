@@ -39,6 +39,15 @@ namespace {
 
 enum : uint32_t { DEAD = 0, FLYING = 1, PENDING = 2 };
 
+// World kinds the kernel is compiled for: the two common layouts get their
+// code width, record width, 24-bit index and bounds table fixed at compile
+// time, so a flight carries no branches on them
+//   WORLD_CODES8    1-byte bounded-density codes, 8-byte records, the
+//                   attenuation bounds table, 24-bit index (CT-sized worlds)
+//   WORLD_PALETTE8  1-byte palette indices, no codes, 24-bit index
+//   WORLD_ANY       every other world (branches at run time)
+enum : uint32_t { WORLD_ANY = 0, WORLD_CODES8 = 1, WORLD_PALETTE8 = 2 };
+
 
 __device__ inline uint32_t rankBelow(uint64_t mask)
 {
@@ -49,12 +58,12 @@ __device__ inline uint32_t rankBelow(uint64_t mask)
 // (the float-to-unsigned conversion clamps negative values to 0); with
 // 24-bit products where the grid allows (the compiler turns a 24-bit
 // multiply-add into a 64-bit one, so the three terms are summed instead)
-__device__ inline uint64_t voxelIndexFast(const World<float>& w, const float* pos, const float* inv)
+__device__ inline uint64_t voxelIndexFast(const World<float>& w, const float* pos, const float* inv, bool index24)
 {
     const uint32_t ix = min(static_cast<uint32_t>((pos[0] - w.extent[0]) * inv[0]), w.nx - 1);
     const uint32_t iy = min(static_cast<uint32_t>((pos[1] - w.extent[2]) * inv[1]), w.ny - 1);
     const uint32_t iz = min(static_cast<uint32_t>((pos[2] - w.extent[4]) * inv[2]), w.nz - 1);
-    if (w.index24)
+    if (index24)
         return __umul24(iz, w.nxy) + __umul24(iy, w.nx) + ix;
     return (static_cast<uint64_t>(iz) * w.ny + iy) * w.nx + ix;
 }
@@ -182,7 +191,7 @@ __device__ inline bool roulette(Particle<float>& p, Rng& rng)
 // (1: unconstrained, 81 VGPRs and 5 waves at L = 0, 1, 91 at L = 2; 6: at
 // most 80 VGPRs, spill-free at L = 0, 1). S: LDS staging level. The
 // autotuner picks among the variants.
-template <int L, int W, uint32_t S>
+template <int L, int W, uint32_t S, uint32_t K>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(W, 8))) void transportPersistent(World<float> w, Tables<float> tg, const Exposure<float>* exps, const uint64_t* expStart,
     uint32_t nExp, uint64_t hBegin, uint64_t hEnd, uint64_t seed, uint64_t firstHistory, Tally tally, Counters* ctr, unsigned long long* next,
     uint32_t regenThreshold, uint32_t interactThreshold, uint32_t poolSize)
@@ -214,6 +223,11 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(W, 8))) voi
     // the photon's place in the LUT; changes only with its energy
     LutPoint<float> lp { 0, 0.0f };
     const float inv[3] = { 1.0f / w.spacing[0], 1.0f / w.spacing[1], 1.0f / w.spacing[2] };
+    // the world's layout, fixed by K where it can be
+    const uint32_t codeBytes = K == WORLD_CODES8 ? 1u : K == WORLD_PALETTE8 ? 0u : w.codeBytes;
+    const uint32_t voxelBytes = K == WORLD_PALETTE8 ? 1u : K == WORLD_CODES8 ? 8u : w.voxelBytes;
+    const bool index24 = K != WORLD_ANY || w.index24 != 0;
+    const bool sumBounds = K == WORLD_CODES8 || (K == WORLD_ANY && t.lutSumBound != nullptr);
     // global index of the launch's first history (PCG increments of queued photons)
     const uint64_t launchFirst = firstHistory + hBegin;
     const uint32_t lane = __lane_id();
@@ -361,7 +375,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(W, 8))) voi
                     for (int i = 0; i < 3; ++i)
                         p.pos[i] += p.dir[i] * step;
                     const bool in = insideFast(w, p.pos);
-                    idx = voxelIndexFast(w, p.pos, inv); // clamped: a valid voxel also outside
+                    idx = voxelIndexFast(w, p.pos, inv, index24); // clamped: a valid voxel also outside
                     Rng r2 = rng;
                     const float u2 = r2.uniform<float>();
                     // The acceptance probability (Σμ·ρ)·(1/μ̂) is monotone in Σμ and
@@ -373,8 +387,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(W, 8))) voi
                     // record).
                     float prob;
                     float dLo, dHi;
-                    if (w.codeBytes != 0) {
-                        const uint32_t code = w.codeBytes == 1 ? static_cast<const uint8_t*>(w.codes)[idx] : static_cast<const uint16_t*>(w.codes)[idx];
+                    if (codeBytes != 0) {
+                        const uint32_t code = codeBytes == 1 ? static_cast<const uint8_t*>(w.codes)[idx] : static_cast<const uint16_t*>(w.codes)[idx];
                         // the interval as one 16-byte load (read field by field,
                         // the compiler splits it into two loads on the chain)
                         const uint4 b = *reinterpret_cast<const uint4*>(w.bounds + code);
@@ -383,13 +397,13 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(W, 8))) voi
                         dHi = __uint_as_float(b.y);
                         density = dLo;
                     } else {
-                        readVoxelPacked(w, idx, matMeas, density);
+                        readVoxelPacked(w, voxelBytes, idx, matMeas, density);
                         dLo = density;
                         dHi = density;
                     }
                     bool settled = false;
                     prob = 0.0f;
-                    if (t.lutSumBound) {
+                    if (sumBounds) {
                         const float2 sb = t.lutSumBound[__umul24(matMeas, t.lutNodes) + lp.k];
                         const float lo = sb.x * dLo * majInv;
                         const float hi = sb.y * dHi * majInv;
@@ -397,7 +411,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(W, 8))) voi
                         prob = lo;
                     }
                     if (!settled) {
-                        if (w.codeBytes != 0)
+                        if (codeBytes != 0)
                             density = __uint_as_float(static_cast<const uint2*>(w.voxels)[idx].x);
                         float att[3];
                         attenuationAt(t, matMeas, lp, att); // 24-bit row product: flag ignored
@@ -444,32 +458,53 @@ using KernelFn = void (*)(World<float>, Tables<float>, const Exposure<float>*, c
 // waves 0: the default, 6 waves per SIMD where that fits 80 VGPRs without
 // spills (L = 0, 1: CT spiral 3.55e9 vs 3.34e9, DX 4.13e9 vs 3.90e9
 // histories/s); L = 2 would spill and keeps 5
-template <int L, uint32_t S>
+template <int L, uint32_t S, uint32_t K>
 KernelFn variant(uint32_t waves)
 {
     if (waves == 0)
         waves = L == 2 ? 5 : 6;
-    return waves >= 6 ? transportPersistent<L, 6, S> : transportPersistent<L, 1, S>;
+    return waves >= 6 ? transportPersistent<L, 6, S, K> : transportPersistent<L, 1, S, K>;
+}
+
+template <int L, uint32_t K>
+KernelFn variant(uint32_t waves, uint32_t stage)
+{
+    return stage >= STAGE_ALL ? variant<L, STAGE_ALL, K>(waves) : stage >= STAGE_SMALL ? variant<L, STAGE_SMALL, K>(waves) : variant<L, STAGE_NONE, K>(waves);
 }
 
 template <int L>
-KernelFn variant(uint32_t waves, uint32_t stage)
+KernelFn variant(uint32_t waves, uint32_t stage, uint32_t kind)
 {
-    return stage >= STAGE_ALL ? variant<L, STAGE_ALL>(waves) : stage >= STAGE_SMALL ? variant<L, STAGE_SMALL>(waves) : variant<L, STAGE_NONE>(waves);
+    return kind == WORLD_CODES8 ? variant<L, WORLD_CODES8>(waves, stage)
+        : kind == WORLD_PALETTE8 ? variant<L, WORLD_PALETTE8>(waves, stage) : variant<L, WORLD_ANY>(waves, stage);
 }
 
-KernelFn kernelFor(int L, uint32_t waves, uint32_t stage)
+KernelFn kernelFor(int L, uint32_t waves, uint32_t stage, uint32_t kind)
 {
-    return L == 0 ? variant<0>(waves, stage) : L == 1 ? variant<1>(waves, stage) : variant<2>(waves, stage);
+    return L == 0 ? variant<0>(waves, stage, kind) : L == 1 ? variant<1>(waves, stage, kind) : variant<2>(waves, stage, kind);
+}
+
+// the kind a world and its tables run as (after launchFp32 has dropped the
+// codes and bounds where they do not pay)
+uint32_t worldKind(const World<float>& w, const Tables<float>& t)
+{
+    if (!w.index24)
+        return WORLD_ANY;
+    if (w.codeBytes == 1 && w.voxelBytes == 8 && t.lutSumBound)
+        return WORLD_CODES8;
+    if (w.codeBytes == 0 && w.voxelBytes == 1 && !t.lutSumBound)
+        return WORLD_PALETTE8;
+    return WORLD_ANY;
 }
 
 }
 
-uint32_t residentBlocksFp32(int L, uint32_t waves, uint32_t ldsPerBlock)
+uint32_t residentBlocksFp32(int L, uint32_t waves, uint32_t ldsPerBlock, uint32_t kind)
 {
     int resident = 0;
     const uint32_t stage = ldsPerBlock ? STAGE_SMALL : STAGE_NONE;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&resident, reinterpret_cast<const void*>(kernelFor(L, waves, stage)), 256, ldsPerBlock) != hipSuccess
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&resident, reinterpret_cast<const void*>(kernelFor(L, waves, stage, kind)), 256, ldsPerBlock)
+            != hipSuccess
         || resident <= 0) {
         hipGetLastError();
         resident = 1;
@@ -529,7 +564,8 @@ void launchFp32(dxmc_ctx* c, const dxmc_run_cfg& cfg, const World<float>& w, con
     const uint32_t stage = stageLevelFp32(cfg, tables);
     const uint32_t lds = ldsBytes(tables, stage, cfg.low_energy_correction) + WAVES_PER_BLOCK * QUEUE_BYTES;
     // default: as many blocks as stay resident on a CU (registers and LDS)
-    const uint32_t perCU = cfg.blocks_per_cu ? cfg.blocks_per_cu : residentBlocksFp32(cfg.low_energy_correction, waves, lds);
+    const uint32_t kind = worldKind(world, tables);
+    const uint32_t perCU = cfg.blocks_per_cu ? cfg.blocks_per_cu : residentBlocksFp32(cfg.low_energy_correction, waves, lds, kind);
     const uint32_t regen = cfg.regeneration_threshold ? std::min<uint32_t>(cfg.regeneration_threshold, 64u) : 4u;
     const uint32_t interact = cfg.interaction_threshold ? std::min<uint32_t>(cfg.interaction_threshold, 64u) : 20u;
     // histories a wave claims per atomic on the global counter
@@ -537,7 +573,7 @@ void launchFp32(dxmc_ctx* c, const dxmc_run_cfg& cfg, const World<float>& w, con
     uint64_t grid = static_cast<uint64_t>(c->numCU) * perCU;
     const uint64_t lanesNeeded = (he - hb + 63) / 64 * 64;
     grid = std::max<uint64_t>(1, std::min<uint64_t>(grid, (lanesNeeded + block - 1) / block));
-    hipLaunchKernelGGL(kernelFor(cfg.low_energy_correction, waves, stage), dim3(static_cast<uint32_t>(grid)), dim3(block), lds, c->stream, world, tables,
+    hipLaunchKernelGGL(kernelFor(cfg.low_energy_correction, waves, stage, kind), dim3(static_cast<uint32_t>(grid)), dim3(block), lds, c->stream, world, tables,
         exps, expStart, nExp, hb, he, cfg.seed, cfg.first_history, tally, ctr, nx, regen, interact, pool);
 }
 }
