@@ -106,10 +106,10 @@ int autotune(dxmc_ctx* c, const dxmc_world* w, const dxmc_tables* t, const dxmc_
     double bestRate = st ? -1 : tr.rate(cfg, st);
     if (!st) {
         if (cfg.precision == DXMC_PRECISION_FP32) {
-            st = searchField(tr, cfg, &dxmc_run_cfg::waves_per_simd, { 5, 6 }, bestRate);
+            st = searchField(tr, cfg, &dxmc_run_cfg::waves_per_simd, { 5, 6, 7 }, bestRate);
             if (!st)
                 st = searchField(tr, cfg, &dxmc_run_cfg::lds_tables, { 1, 2, 3 }, bestRate);
-            const uint32_t resident = residentBlocksFp32(cfg.low_energy_correction, cfg.waves_per_simd, 0);
+            const uint32_t resident = residentBlocksFp32(cfg.low_energy_correction, wavesFp32(cfg.waves_per_simd, cfg.low_energy_correction, 0), 0);
             std::vector<uint32_t> perCU;
             for (uint32_t b : { resident > 1 ? resident - 1 : 1u, std::max(1u, resident / 2) })
                 if (b != resident && std::find(perCU.begin(), perCU.end(), b) == perCU.end())

@@ -369,6 +369,9 @@ int autotune(dxmc_ctx* c, const dxmc_world* w, const dxmc_tables* t, const dxmc_
 // resident blocks per CU of the FP32 kernel (transport_fp32.hip)
 // resident blocks per CU of the FP32 kernel for a world kind (0: any world)
 uint32_t residentBlocksFp32(int L, uint32_t wavesPerSimd, uint32_t ldsPerBlock, uint32_t worldKind = 0);
+// the resident waves per SIMD a run uses (5, 6 or 7) for a requested value
+// (0: the default) and world kind
+uint32_t wavesFp32(uint32_t requested, int L, uint32_t worldKind);
 int runTransport(dxmc_ctx* c, const dxmc_world* w, const dxmc_tables* t, const dxmc_exposure* x, uint64_t nExp, const dxmc_run_cfg& cfg,
     dxmc_tally* tally, const dxmc_run_monitor* monitor, dxmc_run_stats* stats);
 // device inputs of a run's launches: exposure records of both precisions,

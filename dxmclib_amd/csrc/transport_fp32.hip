@@ -165,8 +165,10 @@ constexpr float LN2F = 0.693147180559945309f;
 // loops; a lane that takes the photon only adds the segment offset and the
 // majorant)
 constexpr int QUEUE_FIELDS = 12;
-constexpr uint32_t QUEUE_CAP = 64;
-constexpr uint32_t QUEUE_BYTES = QUEUE_FIELDS * QUEUE_CAP * 4;
+// photons a wave's queue holds: 48 in the 7-wave kernels, whose LDS must
+// fit 7 blocks per CU, else 64 (the whole wave samples at once)
+__host__ __device__ constexpr uint32_t queueCap(uint32_t waves) { return waves >= 7 ? 48u : 64u; }
+__host__ __device__ constexpr uint32_t queueBytes(uint32_t waves) { return QUEUE_FIELDS * queueCap(waves) * 4; }
 constexpr uint32_t WAVES_PER_BLOCK = 4;
 
 // inside the safe extent, without short-circuit branches
@@ -233,7 +235,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(W, 8))) voi
     const uint32_t lane = __lane_id();
 
     // the wave's queue of new photons, already in the world (LDS, SoA)
-    float* q = reinterpret_cast<float*>(lds + ldsBytes(tg, S, L) + (threadIdx.x / 64) * QUEUE_BYTES);
+    constexpr uint32_t QUEUE_CAP = queueCap(W);
+    float* q = reinterpret_cast<float*>(lds + ldsBytes(tg, S, L) + (threadIdx.x / 64) * queueBytes(W));
     uint32_t queued = 0;
 
     for (;;) {
@@ -455,14 +458,14 @@ __global__ void clearCounter(unsigned long long* counter) { *counter = 0; }
 using KernelFn = void (*)(World<float>, Tables<float>, const Exposure<float>*, const uint64_t*, uint32_t, uint64_t, uint64_t, uint64_t, uint64_t,
     Tally, Counters*, unsigned long long*, uint32_t, uint32_t, uint32_t);
 
-// waves 0: the default, 6 waves per SIMD where that fits 80 VGPRs without
-// spills (L = 0, 1: CT spiral 3.55e9 vs 3.34e9, DX 4.13e9 vs 3.90e9
-// histories/s); L = 2 would spill and keeps 5
+// waves: resolved by wavesFp32 (5, 6 or 7); the 7-wave kernel (72 VGPRs)
+// exists for the two fixed world kinds at L = 0, 1
 template <int L, uint32_t S, uint32_t K>
 KernelFn variant(uint32_t waves)
 {
-    if (waves == 0)
-        waves = L == 2 ? 5 : 6;
+    if constexpr (K != WORLD_ANY && L < 2)
+        if (waves >= 7)
+            return transportPersistent<L, 7, S, K>;
     return waves >= 6 ? transportPersistent<L, 6, S, K> : transportPersistent<L, 1, S, K>;
 }
 
@@ -499,6 +502,20 @@ uint32_t worldKind(const World<float>& w, const Tables<float>& t)
 
 }
 
+// Resident waves per SIMD of a run: the requested 5, 6 or 7, or by default
+// 6 where that fits 80 VGPRs without spills (L = 0, 1: CT spiral 3.55e9 vs
+// 3.34e9, DX 4.13e9 vs 3.90e9 histories/s with 5), 7 for palette worlds
+// (DX 4.50e9 vs 4.39e9; CT spiral +0.6 %, whole body -1.3 %, so code worlds
+// keep 6); L = 2 would spill at 6 and keeps 5.
+uint32_t wavesFp32(uint32_t requested, int L, uint32_t kind)
+{
+    if (requested == 0)
+        return L == 2 ? 5u : kind == WORLD_PALETTE8 ? 7u : 6u;
+    if (requested >= 7)
+        return kind != WORLD_ANY && L < 2 ? 7u : 6u;
+    return requested >= 6 ? 6u : 5u;
+}
+
 uint32_t residentBlocksFp32(int L, uint32_t waves, uint32_t ldsPerBlock, uint32_t kind)
 {
     int resident = 0;
@@ -515,11 +532,11 @@ uint32_t residentBlocksFp32(int L, uint32_t waves, uint32_t ldsPerBlock, uint32_
 // staging level of a run: the requested one (lowered until it fits), or by
 // default the attenuation coefficients too when they cost no resident block,
 // else the sampling tables when they cost at most one, else none
-uint32_t stageLevelFp32(const dxmc_run_cfg& cfg, const Tables<float>& t)
+uint32_t stageLevelFp32(const dxmc_run_cfg& cfg, const Tables<float>& t, uint32_t waves, uint32_t kind)
 {
     // a block gets at most 64 KiB of dynamic LDS
     constexpr uint32_t ldsPerBlock = 64 * 1024;
-    auto bytes = [&](uint32_t level) { return uint64_t(ldsBytes(t, level, cfg.low_energy_correction)) + WAVES_PER_BLOCK * QUEUE_BYTES; };
+    auto bytes = [&](uint32_t level) { return uint64_t(ldsBytes(t, level, cfg.low_energy_correction)) + WAVES_PER_BLOCK * queueBytes(waves); };
     auto fits = [&](uint32_t level) { return bytes(level) <= ldsPerBlock; };
     if (cfg.lds_tables >= STAGE_NONE && cfg.lds_tables <= STAGE_ALL) {
         uint32_t level = cfg.lds_tables;
@@ -532,10 +549,8 @@ uint32_t stageLevelFp32(const dxmc_run_cfg& cfg, const Tables<float>& t)
     // 0.98e9 without and 5); the attenuation coefficients are not worth two
     // (CT spiral 2.84e9 vs 2.50e9, DX 4.07e9 vs 3.32e9 with 5 and 3 blocks)
     // and are staged only when they cost none.
-    const uint32_t best = residentBlocksFp32(cfg.low_energy_correction, cfg.waves_per_simd, bytes(STAGE_NONE));
-    auto resident = [&](uint32_t level) {
-        return residentBlocksFp32(cfg.low_energy_correction, cfg.waves_per_simd, static_cast<uint32_t>(bytes(level)));
-    };
+    const uint32_t best = residentBlocksFp32(cfg.low_energy_correction, waves, bytes(STAGE_NONE), kind);
+    auto resident = [&](uint32_t level) { return residentBlocksFp32(cfg.low_energy_correction, waves, static_cast<uint32_t>(bytes(level)), kind); };
     if (fits(STAGE_ALL) && resident(STAGE_ALL) >= best)
         return STAGE_ALL;
     if (fits(STAGE_SMALL) && resident(STAGE_SMALL) + 1 >= best)
@@ -551,7 +566,6 @@ void launchFp32(dxmc_ctx* c, const dxmc_run_cfg& cfg, const World<float>& w, con
     auto* nx = historyCounter;
     hipLaunchKernelGGL(clearCounter, dim3(1), dim3(1), 0, c->stream, nx);
     const uint32_t block = 256;
-    const uint32_t waves = cfg.waves_per_simd;
     World<float> world = w;
     Tables<float> tables = t;
     // the attenuation bounds pay where they also spare the record read
@@ -561,10 +575,11 @@ void launchFp32(dxmc_ctx* c, const dxmc_run_cfg& cfg, const World<float>& w, con
         world.codeBytes = 0;
         tables.lutSumBound = nullptr;
     }
-    const uint32_t stage = stageLevelFp32(cfg, tables);
-    const uint32_t lds = ldsBytes(tables, stage, cfg.low_energy_correction) + WAVES_PER_BLOCK * QUEUE_BYTES;
-    // default: as many blocks as stay resident on a CU (registers and LDS)
     const uint32_t kind = worldKind(world, tables);
+    const uint32_t waves = wavesFp32(cfg.waves_per_simd, cfg.low_energy_correction, kind);
+    const uint32_t stage = stageLevelFp32(cfg, tables, waves, kind);
+    const uint32_t lds = ldsBytes(tables, stage, cfg.low_energy_correction) + WAVES_PER_BLOCK * queueBytes(waves);
+    // default: as many blocks as stay resident on a CU (registers and LDS)
     const uint32_t perCU = cfg.blocks_per_cu ? cfg.blocks_per_cu : residentBlocksFp32(cfg.low_energy_correction, waves, lds, kind);
     const uint32_t regen = cfg.regeneration_threshold ? std::min<uint32_t>(cfg.regeneration_threshold, 64u) : 4u;
     const uint32_t interact = cfg.interaction_threshold ? std::min<uint32_t>(cfg.interaction_threshold, 64u) : 20u;
