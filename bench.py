@@ -45,7 +45,7 @@ def parse():
     p.add_argument("--threshold", type=int, default=0, help="FP32 regeneration threshold (dead lanes per wave), 0: default")
     p.add_argument("--interaction-threshold", type=int, default=0, help="FP32 pending interactions per wave before they run, 0: default")
     p.add_argument("--queue-depth", type=int, default=0, help="FP32 histories per pool refill, 0: default")
-    p.add_argument("--waves-per-simd", type=int, default=0, help="FP32 kernel variant (5, 6), 0: default (6 for L=0, 1; 5 for L=2)")
+    p.add_argument("--waves-per-simd", type=int, default=0, help="FP32 kernel variant (5, 6, 7), 0: default (7 on palette worlds and 6 otherwise for L=0, 1; 5 for L=2)")
     p.add_argument("--lds-tables", type=int, default=0, help="FP32 tables in LDS: 1 none, 2 all but the LUT, 3 all; 0: default")
     p.add_argument("--low-energy-correction", type=int, default=1)
     p.add_argument("--no-autotune", action="store_true", help="skip the autotuner (defaults or the fields given)")
