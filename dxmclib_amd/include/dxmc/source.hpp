@@ -29,8 +29,11 @@
 #include <algorithm>
 #include <array>
 #include <cmath>
+#include <map>
 #include <memory>
+#include <mutex>
 #include <numeric>
+#include <string>
 #include <vector>
 
 namespace dxmc {
@@ -399,6 +402,88 @@ private:
     std::size_t m_totalExposures = 180;
     T m_angleSpan = PI_VAL<T>();
     T m_angleStep = PI_VAL<T>() / T { 180 };
+};
+
+// CTDIw of CT calibration runs, kept per the run's complete input: the
+// exposure records and all tables of the prepared nested transport (tube
+// spectrum, heel and bowtie tables, geometry and weights of every exposure,
+// histories, attenuation and sampling tables), the low-energy model and
+// the phantom diameter. A run is deterministic (fixed seed, one random
+// stream per history), so an equal input gives the same CTDIw; a repeated
+// DOSE-mode CT request then skips its nested transport of 1e8 or more
+// histories. At most 256 entries; an unfinished (cancelled) run is not kept.
+template <Floating T>
+class CalibrationCache {
+public:
+    template <typename Prepared>
+    static std::string key(const Prepared& p, int model, std::uint64_t diameter)
+    {
+        std::string k;
+        auto put = [&k](const void* data, std::size_t bytes) { k.append(static_cast<const char*>(data), bytes); };
+        auto vec = [&put](const auto& v) {
+            const std::uint64_t n = v.size();
+            put(&n, sizeof n);
+            put(v.data(), v.size() * sizeof(v[0]));
+        };
+        put(&model, sizeof model);
+        put(&diameter, sizeof diameter);
+        const TableSet& t = p.tables;
+        vec(t.grid);
+        vec(t.coefficients);
+        vec(t.rita);
+        vec(t.spline);
+        vec(t.spectrumOffset);
+        vec(t.spectrumAlias);
+        vec(t.spectrumProbability);
+        vec(t.spectrumEnergy);
+        vec(t.heelWeight);
+        vec(t.bowtieOffset);
+        vec(t.bowtieAngle);
+        vec(t.bowtieWeight);
+        for (const auto& h : t.heel) {
+            put(&h.energy_start, 4 * sizeof(double));
+            put(&h.energy_size, 3 * sizeof(std::uint32_t));
+        }
+        for (const auto& e : p.exposures) {
+            put(e.position, 18 * sizeof(double)); // position .. mono_energy
+            put(&e.n_histories, sizeof e.n_histories);
+            put(&e.spectrum, 3 * sizeof(std::int32_t));
+        }
+        return k;
+    }
+    static bool find(const std::string& k, T& ctdiw)
+    {
+        std::lock_guard lock(mutex());
+        const auto it = entries().find(k);
+        if (it == entries().end())
+            return false;
+        ctdiw = it->second;
+        return true;
+    }
+    static void store(const std::string& k, T ctdiw)
+    {
+        std::lock_guard lock(mutex());
+        if (entries().size() >= 256)
+            entries().clear();
+        entries()[k] = ctdiw;
+    }
+    static void clear()
+    {
+        std::lock_guard lock(mutex());
+        entries().clear();
+    }
+
+private:
+    static std::mutex& mutex()
+    {
+        static std::mutex m;
+        return m;
+    }
+    static std::map<std::string, T>& entries()
+    {
+        static std::map<std::string, T> e;
+        return e;
+    }
 };
 
 template <Floating T>
@@ -925,20 +1010,26 @@ T CTBaseSource<T>::ctCalibration(U& src, LOWENERGYCORRECTION model, ProgressBar<
     }
     Transport<T> transport;
     transport.setLowEnergyCorrectionModel(model);
-    const auto result = transport(world, &src, progress, false);
-
-    using Hole = typename CTDIPhantom<T>::HolePosition;
-    const std::array<Hole, 5> holes = { Hole::Center, Hole::West, Hole::East, Hole::South, Hole::North };
-    std::array<T, 5> d {};
-    for (std::size_t i = 0; i < 5; ++i) {
-        const auto& idx = world.holeIndices(holes[i]);
-        for (const auto k : idx)
-            d[i] += result.dose[k];
-        d[i] /= idx.size();
+    const auto prep = transport.prepare(world, &src);
+    const std::string key = prep.valid ? CalibrationCache<T>::key(prep, static_cast<int>(model), src.ctdiPhantomDiameter()) : std::string();
+    T ctdiw = 0;
+    if (prep.valid && !CalibrationCache<T>::find(key, ctdiw)) {
+        const auto result = transport.execute(world, &src, prep, progress, false);
+        using Hole = typename CTDIPhantom<T>::HolePosition;
+        const std::array<Hole, 5> holes = { Hole::Center, Hole::West, Hole::East, Hole::South, Hole::North };
+        std::array<T, 5> d {};
+        for (std::size_t i = 0; i < 5; ++i) {
+            const auto& idx = world.holeIndices(holes[i]);
+            for (const auto k : idx)
+                d[i] += result.dose[k];
+            d[i] /= idx.size();
+        }
+        const T peripheral = (d[1] + d[2] + d[3] + d[4]) / T { 4 };
+        ctdiw = (d[0] + 2 * peripheral) / 3;
+        ctdiw *= T { 100 } / src.collimation();
+        if (result.numberOfHistories > 0 && ctdiw > 0)
+            CalibrationCache<T>::store(key, ctdiw);
     }
-    const T peripheral = (d[1] + d[2] + d[3] + d[4]) / T { 4 };
-    T ctdiw = (d[0] + 2 * peripheral) / 3;
-    ctdiw *= T { 100 } / src.collimation();
     return src.ctdiVol() / ctdiw / meanWeight;
 }
 }

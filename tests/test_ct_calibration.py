@@ -7,6 +7,7 @@ CPU: the phantom geometry (disc, holes, measurement map).
 GPU: the calibration factor the host API applies in DOSE mode equals the
 factor restated here from the same phantom transported through the device C
 ABI (identical streams, FP64), and the centre/periphery ratio is physical;
+the factor's CTDIw is cached per the calibration run's input;
 that transport of the CTDI phantom (forced scoring in the holes) equals the
 oracle's on identical streams at reduced histories.
 """
@@ -53,20 +54,24 @@ def test_ctdi_phantom_geometry():
     assert np.array_equal(meas > 0, hole > 0)
 
 
-def _ct_spec(histories_per_exposure, step_deg, ctdi_vol, **kw):
+def _ct_spec(histories_per_exposure, step_deg, ctdi_vol, tube_voltage=120.0, **kw):
     ang, w = workloads.bowtie_profile()
-    return backend.source_spec(capi.SOURCE_CT_AXIAL, tube_voltage=120.0, al_mm=6.0, sdd=1190.0, fov=500.0, collimation=38.4,
+    return backend.source_spec(capi.SOURCE_CT_AXIAL, tube_voltage=tube_voltage, al_mm=6.0, sdd=1190.0, fov=500.0, collimation=38.4,
                                exposure_angle_step_deg=step_deg, histories_per_exposure=histories_per_exposure, bowtie_angles=ang,
                                bowtie_weights=w, heel_effect=1, ctdi_vol=ctdi_vol, **kw)
+
+
+def _target_world():
+    # a small target world: the factor does not depend on it
+    return backend.host_world(np.ones((8, 16, 16), dtype=np.uint8), np.ones((8, 16, 16), dtype=np.float32), (2.0, 2.0, 2.0),
+                              ["Air, Dry (near sea level)", "Water, Liquid"])
 
 
 @pytest.mark.gpu
 def test_ct_calibration_factor(device):
     per, step, ctdivol = 100_000, 10.0, 10.0
     spec = _ct_spec(per, step, ctdivol)
-    # a small target world: the factor does not depend on it
-    tw = backend.host_world(np.ones((8, 16, 16), dtype=np.uint8), np.ones((8, 16, 16), dtype=np.float32), (2.0, 2.0, 2.0),
-                            ["Air, Dry (near sea level)", "Water, Liquid"])
+    tw = _target_world()
     cal = backend.transport(tw, spec, precision=capi.PRECISION_FP64, output_mode=capi.OUTPUT_DOSE, use_source_calibration=True)
     raw = backend.transport(tw, spec, precision=capi.PRECISION_FP64, output_mode=capi.OUTPUT_DOSE, use_source_calibration=False)
     m = raw["dose"] > 0
@@ -107,6 +112,35 @@ def test_ct_calibration_factor(device):
     print(f"calibration factor {factor:.6e}, restated {expected:.6e}; centre/periphery {means[0] / np.mean(means[1:]):.3f}", flush=True)
     assert abs(factor - expected) <= 1e-9 * expected
     assert 0.25 < means[0] / np.mean(means[1:]) < 0.9
+
+
+@pytest.mark.gpu
+def test_ct_calibration_cached(device):
+    """The CTDIw of a calibration run is kept per the run's complete input
+    (exposures and tables of the nested transport, model, phantom): a
+    repeated DOSE-mode call reuses it (the same factor in a fraction of the
+    time), another CTDIvol scales the factor exactly, another tube voltage
+    runs a calibration of its own."""
+    import time
+    tw = _target_world()
+
+    def factor(spec):
+        t0 = time.perf_counter()
+        cal = backend.transport(tw, spec, precision=capi.PRECISION_FP64, output_mode=capi.OUTPUT_DOSE, use_source_calibration=True)
+        dt = time.perf_counter() - t0
+        raw = backend.transport(tw, spec, precision=capi.PRECISION_FP64, output_mode=capi.OUTPUT_DOSE, use_source_calibration=False)
+        m = raw["dose"] > 0
+        return float(np.median(cal["dose"][m] / raw["dose"][m])), dt
+
+    # settings no other test uses, so the first call is not already cached
+    f1, t1 = factor(_ct_spec(50_000, 12.0, 10.0))
+    f2, t2 = factor(_ct_spec(50_000, 12.0, 10.0))
+    f3, _ = factor(_ct_spec(50_000, 12.0, 25.0))
+    f4, t4 = factor(_ct_spec(50_000, 12.0, 10.0, tube_voltage=100.0))
+    print(f"factor {f1:.6e} in {t1:.3f} s, again {f2:.6e} in {t2:.3f} s; CTDIvol x2.5 {f3 / f1:.12f}; 100 kV {f4:.6e} in {t4:.3f} s", flush=True)
+    assert f2 == pytest.approx(f1, rel=1e-12) and t2 < 0.5 * t1
+    assert f3 == pytest.approx(2.5 * f1, rel=1e-12)
+    assert t4 > 2 * t2 and abs(f4 / f1 - 1) > 1e-3
 
 
 @pytest.mark.gpu
