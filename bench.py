@@ -332,7 +332,7 @@ def main():
             # bound: the roofline the kernel is priced against (algorithmic HBM bytes); limiter: what the counters
             # show holds it below that roof (vector-ALU issue, not bandwidth)
             "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK, "unit": "GB/s", "frac": achieved / HBM_PEAK,
-                         "traffic": None, "traffic_unit": "GB per launch", "traffic_source": None, "limiter": "valu_issue"},
+                         "traffic": None, "traffic_unit": "GB per launch", "traffic_source": None, "limiter": "valu_issue_and_gather_latency"},
             "counters": {"steps_per_history": steps / histories, "events_per_history": events / histories,
                          "kernel_ms_per_launch": kernel_ms / max(launches, 1), "launches_per_step": launches / args.steps},
             "setup_seconds": setup_s,

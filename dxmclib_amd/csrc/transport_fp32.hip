@@ -15,10 +15,11 @@
 // So the long, branchy interaction samplers (Compton/Rayleigh rejection
 // loops, fluorescence) run with many lanes active instead of with the few
 // lanes that happen to interact in a given flight, while the cheap flights
-// run nearly full. New photons are sampled by the whole wave at once (64
-// histories from a wave-private pool of history indices, which one lane
-// refills with a single atomic on the global counter) into a per-wave queue
-// in LDS; dead lanes take photons from that queue once at least
+// run nearly full. New photons are sampled by the whole wave at once (up to
+// the queue's 64 or 48 histories from a wave-private pool of history
+// indices, which one lane refills with a single atomic on the global
+// counter) into a per-wave queue in LDS; dead lanes take photons from that
+// queue once at least
 // 'regeneration_threshold' of them wait, each by its rank among the dead
 // lanes (ballot + mbcnt). So the costly source sampling runs with all lanes
 // active and a lane is refilled with a few LDS reads.
@@ -29,8 +30,10 @@
 // ran a history nor on the grid size or thresholds.
 //
 // Arithmetic is FP32 (base-2 anchored LUT, FP32 transcendentals, voxel index
-// by multiplication with the reciprocal spacing); each scoring event is added
-// to the FP64 tallies with hardware FP64 atomics.
+// by multiplication with the reciprocal spacing, FP32 denormals flushed);
+// each scoring event is added to the FP64 tallies with hardware FP64
+// atomics. The kernel is compiled per low-energy model, register budget
+// (5, 6 or 7 waves per SIMD), LDS staging level and world kind.
 #include "internal.hpp"
 
 using namespace dxmc_dev;
@@ -190,9 +193,10 @@ __device__ inline bool roulette(Particle<float>& p, Rng& rng)
 }
 
 // W: minimum resident waves per SIMD the register allocation must allow
-// (1: unconstrained, 81 VGPRs and 5 waves at L = 0, 1, 91 at L = 2; 6: at
-// most 80 VGPRs, spill-free at L = 0, 1). S: LDS staging level. The
-// autotuner picks among the variants.
+// (1: unconstrained, 5 waves; 6: at most 80 VGPRs, spill-free at L = 0, 1;
+// 7: at most 72 VGPRs, spill-free for the two fixed world kinds at L = 0, 1,
+// with a 48-photon queue). S: LDS staging level. K: world kind. The
+// autotuner picks among the register budgets and staging levels.
 template <int L, int W, uint32_t S, uint32_t K>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(W, 8))) void transportPersistent(World<float> w, Tables<float> tg, const Exposure<float>* exps, const uint64_t* expStart,
     uint32_t nExp, uint64_t hBegin, uint64_t hEnd, uint64_t seed, uint64_t firstHistory, Tally tally, Counters* ctr, unsigned long long* next,
