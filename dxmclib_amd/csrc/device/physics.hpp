@@ -132,7 +132,13 @@ struct Exposure {
 // voxel's material and an interval holding its density ('bounds'): the
 // Woodcock decision is settled from the interval at most steps, and the
 // 8-byte record is read only when the draw falls inside the interval's
-// acceptance band.
+// acceptance band. A world with 1-byte codes also carries 'blocks': per
+// block of 8x8x8 voxels (clipped at the grid's upper faces) the smallest
+// code | the largest code << 8 when all its voxels share one material (the
+// codes of a material are ordered by density), else BLOCK_MIXED, which
+// settles nothing (its code MIXED_CODE is no voxel's). The block's interval
+// settles most decisions without the voxel's code; the code array is read
+// only for the draws that fall inside the block's acceptance band.
 struct DensityBound {
     float lo, hi; // lo <= FP32 density <= hi for every voxel with this code
     uint32_t material, pad;
@@ -154,7 +160,14 @@ struct World {
     // 1 when the linear voxel index fits 24-bit products: nx * ny (the
     // slice stride 'nxy') and nz below 2^24 and the voxel count below 2^32
     uint32_t index24, nxy;
+    // block codes (null without 1-byte codes), blocks per row and per slice
+    const uint16_t* blocks;
+    uint32_t bnx, bnxy;
 };
+
+constexpr uint32_t BLOCK_SHIFT = 3; // 8 voxels per block edge
+constexpr uint32_t MIXED_CODE = 255; // the 1-byte code kept for mixed blocks
+constexpr uint16_t BLOCK_MIXED = MIXED_CODE | MIXED_CODE << 8;
 
 // One voxel's accumulators in 32 aligned bytes: the three atomics of a
 // scoring event land in one cache line (separate arrays cost three random

@@ -220,6 +220,10 @@ struct dxmc_world {
     void* bounds = nullptr;
     uint32_t codeBytes = 0;
     uint32_t nCodes = 0;
+    // 1-byte code worlds: a 2-byte code per block of 8x8x8 voxels (smallest
+    // | largest code << 8 of a one-material block, else BLOCK_MIXED)
+    void* blocks = nullptr;
+    uint32_t blockDims[3] = { 0, 0, 0 };
     uint64_t bytes = 0;
 };
 
@@ -327,6 +331,9 @@ inline dxmc_dev::World<T> deviceWorld(const dxmc_world* w)
     d.codeBytes = w->codeBytes;
     d.index24 = uint64_t(d.nx) * d.ny < (uint64_t(1) << 24) && d.nz < (uint32_t(1) << 24) && uint64_t(d.nx) * d.ny * d.nz < (uint64_t(1) << 32);
     d.nxy = d.index24 ? d.nx * d.ny : 0;
+    d.blocks = static_cast<const uint16_t*>(w->blocks);
+    d.bnx = w->blockDims[0];
+    d.bnxy = w->blockDims[0] * w->blockDims[1];
     d.voxelBytes = w->voxelBytes;
     return d;
 }

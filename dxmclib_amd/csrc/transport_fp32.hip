@@ -45,8 +45,9 @@ enum : uint32_t { DEAD = 0, FLYING = 1, PENDING = 2 };
 // World kinds the kernel is compiled for: the two common layouts get their
 // code width, record width, 24-bit index and bounds table fixed at compile
 // time, so a flight carries no branches on them
-//   WORLD_CODES8    1-byte bounded-density codes, 8-byte records, the
-//                   attenuation bounds table, 24-bit index (CT-sized worlds)
+//   WORLD_CODES8    1-byte bounded-density codes and their block codes,
+//                   8-byte records, the attenuation bounds table, 24-bit
+//                   index (CT-sized worlds)
 //   WORLD_PALETTE8  1-byte palette indices, no codes, 24-bit index
 //   WORLD_ANY       every other world (branches at run time)
 enum : uint32_t { WORLD_ANY = 0, WORLD_CODES8 = 1, WORLD_PALETTE8 = 2 };
@@ -61,14 +62,23 @@ __device__ inline uint32_t rankBelow(uint64_t mask)
 // (the float-to-unsigned conversion clamps negative values to 0); with
 // 24-bit products where the grid allows (the compiler turns a 24-bit
 // multiply-add into a 64-bit one, so the three terms are summed instead)
-__device__ inline uint64_t voxelIndexFast(const World<float>& w, const float* pos, const float* inv, bool index24)
+__device__ inline uint3 voxelCoordsFast(const World<float>& w, const float* pos, const float* inv)
 {
-    const uint32_t ix = min(static_cast<uint32_t>((pos[0] - w.extent[0]) * inv[0]), w.nx - 1);
-    const uint32_t iy = min(static_cast<uint32_t>((pos[1] - w.extent[2]) * inv[1]), w.ny - 1);
-    const uint32_t iz = min(static_cast<uint32_t>((pos[2] - w.extent[4]) * inv[2]), w.nz - 1);
+    return make_uint3(min(static_cast<uint32_t>((pos[0] - w.extent[0]) * inv[0]), w.nx - 1),
+        min(static_cast<uint32_t>((pos[1] - w.extent[2]) * inv[1]), w.ny - 1), min(static_cast<uint32_t>((pos[2] - w.extent[4]) * inv[2]), w.nz - 1));
+}
+
+__device__ inline uint64_t voxelIndexFast(const World<float>& w, uint3 v, bool index24)
+{
     if (index24)
-        return __umul24(iz, w.nxy) + __umul24(iy, w.nx) + ix;
-    return (static_cast<uint64_t>(iz) * w.ny + iy) * w.nx + ix;
+        return __umul24(v.z, w.nxy) + __umul24(v.y, w.nx) + v.x;
+    return (static_cast<uint64_t>(v.z) * w.ny + v.y) * w.nx + v.x;
+}
+
+// block of a voxel (1-byte code worlds: 24-bit products hold)
+__device__ inline uint32_t blockIndexFast(const World<float>& w, uint3 v)
+{
+    return __umul24(v.z >> BLOCK_SHIFT, w.bnxy) + __umul24(v.y >> BLOCK_SHIFT, w.bnx) + (v.x >> BLOCK_SHIFT);
 }
 
 // ------------------------------------------------------ LDS-staged tables
@@ -382,47 +392,62 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(W, 8))) voi
                     for (int i = 0; i < 3; ++i)
                         p.pos[i] += p.dir[i] * step;
                     const bool in = insideFast(w, p.pos);
-                    idx = voxelIndexFast(w, p.pos, inv, index24); // clamped: a valid voxel also outside
+                    const uint3 vc = voxelCoordsFast(w, p.pos, inv); // clamped: a valid voxel also outside
+                    idx = voxelIndexFast(w, vc, index24);
                     Rng r2 = rng;
                     const float u2 = r2.uniform<float>();
                     // The acceptance probability (Σμ·ρ)·(1/μ̂) is monotone in Σμ and
                     // in ρ. Bounds of both settle most draws exactly as the values
                     // would: u2 < p(lower bounds) accepts, u2 >= p(upper bounds)
                     // rejects. Σμ is bounded over the photon's LUT segment; ρ by the
+                    // voxel's block when the block holds one material, else by the
                     // voxel's code when the world has codes, else read exactly. Only
-                    // a draw between the two evaluates the attenuation (and reads the
-                    // record).
-                    float prob;
-                    float dLo, dHi;
-                    if (codeBytes != 0) {
-                        const uint32_t code = codeBytes == 1 ? static_cast<const uint8_t*>(w.codes)[idx] : static_cast<const uint16_t*>(w.codes)[idx];
-                        // the interval as one 16-byte load (read field by field,
-                        // the compiler splits it into two loads on the chain)
-                        const uint4 b = *reinterpret_cast<const uint4*>(w.bounds + code);
-                        matMeas = b.z; // material; code worlds carry no measurement flags
-                        dLo = __uint_as_float(b.x);
-                        dHi = __uint_as_float(b.y);
-                        density = dLo;
-                    } else {
-                        readVoxelPacked(w, voxelBytes, idx, matMeas, density);
-                        dLo = density;
-                        dHi = density;
-                    }
+                    // a draw between the bounds evaluates the attenuation (and reads
+                    // the record).
+                    float prob = 0.0f;
                     bool settled = false;
-                    prob = 0.0f;
-                    if (sumBounds) {
-                        const float2 sb = t.lutSumBound[__umul24(matMeas, t.lutNodes) + lp.k];
-                        const float lo = sb.x * dLo * majInv;
-                        const float hi = sb.y * dHi * majInv;
-                        settled = (u2 < lo) | (u2 >= hi);
+                    if (K == WORLD_CODES8) {
+                        const uint32_t bc = w.blocks[blockIndexFast(w, vc)];
+                        const float bLo = w.bounds[bc & 0xffu].lo;
+                        const uint2 bh = *reinterpret_cast<const uint2*>(&w.bounds[bc >> 8].hi); // hi, material
+                        const float2 sb = t.lutSumBound[__umul24(bh.y, t.lutNodes) + lp.k];
+                        const float lo = sb.x * bLo * majInv;
+                        const float hi = sb.y * __uint_as_float(bh.x) * majInv;
+                        settled = (bc != BLOCK_MIXED) & ((u2 < lo) | (u2 >= hi));
+                        matMeas = bh.y;
+                        density = bLo;
                         prob = lo;
                     }
                     if (!settled) {
-                        if (codeBytes != 0)
-                            density = __uint_as_float(static_cast<const uint2*>(w.voxels)[idx].x);
-                        float att[3];
-                        attenuationAt(t, matMeas, lp, att); // 24-bit row product: flag ignored
-                        prob = ((att[0] + att[1]) + att[2]) * density * majInv;
+                        float dLo, dHi;
+                        if (codeBytes != 0) {
+                            const uint32_t code = codeBytes == 1 ? static_cast<const uint8_t*>(w.codes)[idx] : static_cast<const uint16_t*>(w.codes)[idx];
+                            // the interval as one 16-byte load (read field by field,
+                            // the compiler splits it into two loads on the chain)
+                            const uint4 b = *reinterpret_cast<const uint4*>(w.bounds + code);
+                            matMeas = b.z; // material; code worlds carry no measurement flags
+                            dLo = __uint_as_float(b.x);
+                            dHi = __uint_as_float(b.y);
+                            density = dLo;
+                        } else {
+                            readVoxelPacked(w, voxelBytes, idx, matMeas, density);
+                            dLo = density;
+                            dHi = density;
+                        }
+                        if (sumBounds) {
+                            const float2 sb = t.lutSumBound[__umul24(matMeas, t.lutNodes) + lp.k];
+                            const float lo = sb.x * dLo * majInv;
+                            const float hi = sb.y * dHi * majInv;
+                            settled = (u2 < lo) | (u2 >= hi);
+                            prob = lo;
+                        }
+                        if (!settled) {
+                            if (codeBytes != 0)
+                                density = __uint_as_float(static_cast<const uint2*>(w.voxels)[idx].x);
+                            float att[3];
+                            attenuationAt(t, matMeas, lp, att); // 24-bit row product: flag ignored
+                            prob = ((att[0] + att[1]) + att[2]) * density * majInv;
+                        }
                     }
                     const bool measured = matMeas > 0xffffffu;
                     const bool drawn = in & !measured;
@@ -497,7 +522,7 @@ uint32_t worldKind(const World<float>& w, const Tables<float>& t)
 {
     if (!w.index24)
         return WORLD_ANY;
-    if (w.codeBytes == 1 && w.voxelBytes == 8 && t.lutSumBound)
+    if (w.codeBytes == 1 && w.voxelBytes == 8 && t.lutSumBound && w.blocks)
         return WORLD_CODES8;
     if (w.codeBytes == 0 && w.voxelBytes == 1 && !t.lutSumBound)
         return WORLD_PALETTE8;

@@ -17,6 +17,10 @@
 //   boundCodes<I>   for those worlds, a 1- or 2-byte code per voxel naming
 //                   its material and a density interval (Woodcock decisions
 //                   settled from the interval at most steps)
+//   blockCodes      with 1-byte codes, the code range of every block of
+//                   8x8x8 voxels that holds one material (decisions settled
+//                   from the block's interval without reading the voxel's
+//                   code)
 // The host reads back a few hundred bytes (maxima, flags, distinct count),
 // so the cost of an upload is the copy and a few HBM passes.
 #include "dxmc_hip.h"
@@ -266,6 +270,41 @@ __global__ __launch_bounds__(SCAN_BLOCK) void boundCodes(const uint2* __restrict
     }
 }
 
+// Block code of every block of 8x8x8 voxels of a 1-byte code world (the
+// blocks of the last row, column and slice clipped to the grid): smallest
+// code | largest code << 8 when every voxel of the block has the same
+// material, else BLOCK_MIXED. A material's codes are its density bins in
+// ascending order, so the interval [lo of the smallest, hi of the largest]
+// holds every density of the block.
+__global__ __launch_bounds__(SCAN_BLOCK) void blockCodes(const uint8_t* __restrict__ codes, const dxmc_dev::DensityBound* __restrict__ bounds,
+    uint32_t nx, uint32_t ny, uint32_t nz, uint32_t bnx, uint32_t bny, uint64_t nBlocks, uint16_t* __restrict__ blocks)
+{
+    constexpr uint32_t E = 1u << dxmc_dev::BLOCK_SHIFT;
+    const uint64_t stride = static_cast<uint64_t>(gridDim.x) * SCAN_BLOCK;
+    for (uint64_t b = static_cast<uint64_t>(blockIdx.x) * SCAN_BLOCK + threadIdx.x; b < nBlocks; b += stride) {
+        const uint32_t bx = static_cast<uint32_t>(b % bnx);
+        const uint32_t by = static_cast<uint32_t>((b / bnx) % bny);
+        const uint32_t bz = static_cast<uint32_t>(b / (static_cast<uint64_t>(bnx) * bny));
+        const uint32_t x0 = bx * E, y0 = by * E, z0 = bz * E;
+        const uint32_t x1 = min(x0 + E, nx), y1 = min(y0 + E, ny), z1 = min(z0 + E, nz);
+        const uint32_t first = codes[(static_cast<uint64_t>(z0) * ny + y0) * nx + x0];
+        const uint32_t material = bounds[first].material;
+        uint32_t lo = first, hi = first;
+        bool mixed = false;
+        for (uint32_t z = z0; z < z1; ++z)
+            for (uint32_t y = y0; y < y1; ++y) {
+                const uint8_t* row = codes + (static_cast<uint64_t>(z) * ny + y) * nx;
+                for (uint32_t x = x0; x < x1; ++x) {
+                    const uint32_t c = row[x];
+                    mixed |= bounds[c].material != material;
+                    lo = min(lo, c);
+                    hi = max(hi, c);
+                }
+            }
+        blocks[b] = mixed ? dxmc_dev::BLOCK_MIXED : static_cast<uint16_t>(lo | hi << 8);
+    }
+}
+
 // Scan results read back by the host.
 struct ScanResult {
     unsigned long long maxKey[SCAN_BLOCK];
@@ -291,11 +330,12 @@ float floatAbove(double v)
 }
 
 // Bounded-density codes of an 8-byte-record world without measurement
-// flags: the materials present share the 256 (1-byte codes, up to 64
-// materials) or 65536 (2-byte codes) codes equally; each material's
-// [min, max] density range is cut into that many equal bins, each widened
-// outwards to FP32 values. Leaves codeBytes 0 when a voxel's density does
-// not fall in the bin computed for it.
+// flags: the materials present share the 255 (1-byte codes, up to 64
+// materials; code 255 is kept for mixed blocks) or 65536 (2-byte codes)
+// codes equally; each material's [min, max] density range is cut into that
+// many equal bins, each widened outwards to FP32 values. Leaves codeBytes 0
+// when a voxel's density does not fall in the bin computed for it. With
+// 1-byte codes the block codes follow.
 int buildBoundCodes(dxmc_ctx* c, dxmc_world* w, ScanResult* dev, const ScanResult& host)
 {
     std::vector<uint32_t> present;
@@ -306,8 +346,10 @@ int buildBoundCodes(dxmc_ctx* c, dxmc_world* w, ScanResult* dev, const ScanResul
         return DXMC_OK;
     const uint32_t codeBytes = present.size() <= 64 ? 1u : 2u;
     const uint32_t nCodes = codeBytes == 1 ? 256u : 65536u;
-    const uint32_t bins = nCodes / static_cast<uint32_t>(present.size());
+    const uint32_t bins = (codeBytes == 1 ? nCodes - 1 : nCodes) / static_cast<uint32_t>(present.size());
     std::vector<dxmc_dev::DensityBound> table(nCodes, dxmc_dev::DensityBound { 1.0f, 0.0f, 0xffffffffu, 0u });
+    if (codeBytes == 1) // a valid material row; the kernels test BLOCK_MIXED before its interval
+        table[dxmc_dev::MIXED_CODE] = dxmc_dev::DensityBound { 0.0f, 0.0f, 0u, 0u };
     std::vector<uint32_t> base(SCAN_BLOCK, 0);
     std::vector<double> lower(SCAN_BLOCK, 0.0), scale(SCAN_BLOCK, 0.0);
     for (uint32_t j = 0; j < present.size(); ++j) {
@@ -353,6 +395,21 @@ int buildBoundCodes(dxmc_ctx* c, dxmc_world* w, ScanResult* dev, const ScanResul
     }
     w->codeBytes = codeBytes;
     w->nCodes = nCodes;
+    if (codeBytes != 1)
+        return DXMC_OK;
+    constexpr uint32_t E = 1u << dxmc_dev::BLOCK_SHIFT;
+    uint32_t bd[3];
+    for (int i = 0; i < 3; ++i)
+        bd[i] = (w->dims[i] + E - 1) / E;
+    const uint64_t nBlocks = uint64_t(bd[0]) * bd[1] * bd[2];
+    if ((st = c->alloc(&w->blocks, nBlocks * sizeof(uint16_t))))
+        return st;
+    hipLaunchKernelGGL(blockCodes, dim3(gridFor(nBlocks, SCAN_BLOCK, SCAN_GRID)), block, 0, c->stream, static_cast<const uint8_t*>(w->codes), bnd,
+        w->dims[0], w->dims[1], w->dims[2], bd[0], bd[1], nBlocks, static_cast<uint16_t*>(w->blocks));
+    if ((st = c->check(hipGetLastError())))
+        return st;
+    for (int i = 0; i < 3; ++i)
+        w->blockDims[i] = bd[i];
     return DXMC_OK;
 }
 
@@ -473,7 +530,7 @@ int dxmc_hip_world_upload(dxmc_ctx* c, const dxmc_world_desc* d, dxmc_world** ou
     if ((st = fillVoxelRecords(c, w, res, host)))
         return fail(st);
     w->bytes = n * (1 + sizeof(double) + w->voxelBytes + (w->hasMeasurement ? 1 : 0) + w->codeBytes) + w->nPalette * sizeof(uint2)
-        + w->nCodes * sizeof(dxmc_dev::DensityBound);
+        + w->nCodes * sizeof(dxmc_dev::DensityBound) + uint64_t(w->blockDims[0]) * w->blockDims[1] * w->blockDims[2] * sizeof(uint16_t);
     *out = w;
     return DXMC_OK;
 }
@@ -495,6 +552,7 @@ int dxmc_hip_world_destroy(dxmc_ctx* c, dxmc_world* w)
     c->release(w->palette);
     c->release(w->codes);
     c->release(w->bounds);
+    c->release(w->blocks);
     hipStreamSynchronize(c->stream);
     delete w;
     return DXMC_OK;

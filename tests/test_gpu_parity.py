@@ -240,6 +240,12 @@ def _spectrum(spec):
     return w, e
 
 
+def _block_bytes(shape):
+    """Bytes of the 2-byte block codes a 1-byte code world carries: one per
+    block of 8x8x8 voxels, the last row, column and slice clipped."""
+    return 2 * int(np.prod([-(-s // 8) for s in shape]))
+
+
 def _zstats(d1, v1, d2, v2):
     m = (d1 > 0) | (d2 > 0)
     z = (d1[m] - d2[m]) / np.sqrt(np.maximum(v1[m] + v2[m], 1e-300))
@@ -339,7 +345,7 @@ def test_fp32_voxel_records(device, records):
     w = device.upload_world(mat, dens, wl.spacing, ext, 2)
     try:
         n = mat.size
-        assert device.world_bytes(w) == n * (1 + 8 + vb) + 8 * distinct + (16 * 256 if records == "direct" else 0)
+        assert device.world_bytes(w) == n * (1 + 8 + vb) + 8 * distinct + (16 * 256 + _block_bytes(shape) if records == "direct" else 0)
     finally:
         device.destroy_world(w)
     d64, _, v64, s64, *_ = _run_device(device, wl, 1, capi.PRECISION_FP64)
@@ -377,7 +383,7 @@ def test_bounded_steps_exact(device, n_materials, jitter):
     code_bytes = 1 if n_materials <= 64 else 2
     n = mat.size
     if jitter:
-        assert device.world_bytes(w) == n * (1 + 8 + 8 + code_bytes) + 16 * (256 if code_bytes == 1 else 65536)
+        assert device.world_bytes(w) == n * (1 + 8 + 8 + code_bytes) + (16 * 256 + _block_bytes(shape) if code_bytes == 1 else 16 * 65536)
     else:
         assert device.world_bytes(w) == n * (1 + 8 + 1) + 8 * n_materials
     maxd = device.world_max_density(w, n_materials)
@@ -408,6 +414,62 @@ def test_bounded_steps_exact(device, n_materials, jitter):
         assert np.array_equal(o[3], out[0][3])
         # FP64 atomics add in run-dependent order: rounding only
         assert np.allclose(o[2], out[0][2], rtol=1e-10, atol=0)
+
+
+def test_block_bounds_exact(device):
+    """Worlds with 1-byte codes also carry a code range per block of 8x8x8
+    voxels: one-material blocks settle most Woodcock decisions from the
+    block's density interval, without the voxel's code. A world of air, a
+    water slab with a bone ellipsoid and a lung region (blocks of one
+    material, blocks on the boundaries holding several, ragged sizes that
+    clip the last blocks) runs the same histories as evaluating every step
+    in full."""
+    print("block bounds", flush=True)
+    rng = np.random.default_rng(21)
+    shape = (45, 50, 53)  # z, y, x: no size a multiple of 8
+    z, y, x = np.meshgrid(*[np.arange(s) for s in shape], indexing="ij")
+    mat = np.zeros(shape, np.uint8)
+    mat[8:40, 5:45, :] = 1
+    mat[((z - 24) / 9.0) ** 2 + ((y - 25) / 12.0) ** 2 + ((x - 20) / 10.0) ** 2 <= 1.0] = 2
+    mat[10:30, 10:26, 36:50] = 3
+    dens = (np.array([0.001205, 1.0, 1.92, 0.26], np.float32)[mat] * rng.uniform(0.98, 1.02, shape)).astype(np.float32)
+    spacing = (1.0, 1.0, 1.0)
+    ext = backend.world_extent(shape[::-1], spacing)
+    names = [workloads.AIR, workloads.WATER, workloads.BONE, workloads.LUNG]
+    w = device.upload_world(mat, dens, spacing, ext, 4)
+    n = mat.size
+    assert device.world_bytes(w) == n * (1 + 8 + 8 + 1) + 16 * 256 + _block_bytes(shape)
+    maxd = device.world_max_density(w, 4)
+    t = backend.HostTables(names, maxd, 120.0, 1.0)
+    spec = backend.source_spec(capi.SOURCE_DX, tube_voltage=120.0, al_mm=2.5, sdd=1000.0, field_size=(60.0, 60.0),
+                               histories_per_exposure=25_000, total_exposures=8)
+    exps = t.exposures(spec)
+    tt = device.upload_tables(t.desc())
+    out = []
+    try:
+        for L in (1, 2):
+            for flags, waves in ((capi.RUN_COUNTERS, 0), (capi.RUN_COUNTERS | capi.RUN_EXACT_STEPS, 0), (capi.RUN_COUNTERS, 7)):
+                if L == 2 and waves == 7:
+                    continue
+                cfg = capi.RunCfg()
+                cfg.flags = flags
+                cfg.waves_per_simd = waves
+                tally = device.tally(n)
+                stats = device.run(w, tt, exps, tally, capi.PRECISION_FP32, L, seed=5, tuning=cfg)
+                d, ev, v = device.download(tally, n)
+                device.destroy_tally(tally)
+                out.append((L, stats.steps, stats.events, d, ev))
+    finally:
+        device.destroy_tables(tt)
+        device.destroy_world(w)
+    print(f"  steps {[o[1] for o in out]}, events {[o[2] for o in out]}")
+    for L in (1, 2):
+        runs = [o for o in out if o[0] == L]
+        assert runs[0][1] > 0 and runs[0][2] > 0
+        for o in runs[1:]:
+            assert o[1] == runs[0][1] and o[2] == runs[0][2]
+            assert np.array_equal(o[4], runs[0][4])
+            assert np.allclose(o[3], runs[0][3], rtol=1e-10, atol=0)
 
 
 @pytest.mark.parametrize("axis", [0, 1, 2])
@@ -477,7 +539,7 @@ def test_world_scan_on_device(device, dtype):
     w = upload()
     try:
         assert np.array_equal(device.world_max_density(w, 5), want)
-        assert device.world_bytes(w) == n * (1 + 8 + 8 + 1) + 16 * 256
+        assert device.world_bytes(w) == n * (1 + 8 + 8 + 1) + 16 * 256 + _block_bytes(shape)
     finally:
         device.destroy_world(w)
     # an all-zero measurement map is dropped; one flag keeps it (and then
@@ -485,7 +547,7 @@ def test_world_scan_on_device(device, dtype):
     meas = np.zeros(shape, np.uint8)
     w = upload(meas)
     try:
-        assert device.world_bytes(w) == n * (1 + 8 + 8 + 1) + 16 * 256
+        assert device.world_bytes(w) == n * (1 + 8 + 8 + 1) + 16 * 256 + _block_bytes(shape)
     finally:
         device.destroy_world(w)
     meas[3, 4, 5] = 1
