@@ -17,6 +17,7 @@ restatement of the reference method, on the host cores on a bounded sample).
 from __future__ import annotations
 
 import argparse
+import glob
 import json
 import os
 import sys
@@ -45,7 +46,7 @@ def parse():
     p.add_argument("--threshold", type=int, default=0, help="FP32 regeneration threshold (dead lanes per wave), 0: default")
     p.add_argument("--interaction-threshold", type=int, default=0, help="FP32 pending interactions per wave before they run, 0: default")
     p.add_argument("--queue-depth", type=int, default=0, help="FP32 histories per pool refill, 0: default")
-    p.add_argument("--waves-per-simd", type=int, default=0, help="FP32 kernel variant (5, 6, 7), 0: default (7 on palette worlds and 6 otherwise for L=0, 1; 5 for L=2)")
+    p.add_argument("--waves-per-simd", type=int, default=0, help="FP32 kernel variant (5, 6, 7), 0: default (7 on worlds of 1-byte codes or palette indices and 6 on others for L=0, 1; 5 for L=2)")
     p.add_argument("--lds-tables", type=int, default=0, help="FP32 tables in LDS: 1 none, 2 all but the LUT, 3 all; 0: default")
     p.add_argument("--low-energy-correction", type=int, default=1)
     p.add_argument("--no-autotune", action="store_true", help="skip the autotuner (defaults or the fields given)")
@@ -221,15 +222,15 @@ def measured_traffic(workload, histories_per_launch):
     copy-kernel calibration), per history times this run's launch size, with
     where the figure comes from; None when no profile of the workload is
     committed."""
-    path = os.path.join(ROOT, "profiles", "transport_fp32_ct_spiral.json")
-    if not os.path.exists(path):
-        return None
-    with open(path) as f:
-        prof = json.load(f)
-    if prof.get("workload") != workload:
-        return None
-    per = prof["fetch_bytes_per_history"] + prof["write_bytes_per_history"]
-    return per * histories_per_launch / 1e9, "rocprofv3 FETCH_SIZE+WRITE_SIZE per history (profiles/transport_fp32_ct_spiral.json) x histories per launch"
+    for path in sorted(glob.glob(os.path.join(ROOT, "profiles", "transport_fp32_*.json"))):
+        with open(path) as f:
+            prof = json.load(f)
+        if prof.get("workload") != workload:
+            continue
+        per = prof["fetch_bytes_per_history"] + prof["write_bytes_per_history"]
+        name = os.path.basename(path)
+        return per * histories_per_launch / 1e9, f"rocprofv3 FETCH_SIZE+WRITE_SIZE per history (profiles/{name}) x histories per launch"
+    return None
 
 
 def spectrum_of(spec):

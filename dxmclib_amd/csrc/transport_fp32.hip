@@ -533,13 +533,16 @@ uint32_t worldKind(const World<float>& w, const Tables<float>& t)
 
 // Resident waves per SIMD of a run: the requested 5, 6 or 7, or by default
 // 6 where that fits 80 VGPRs without spills (L = 0, 1: CT spiral 3.55e9 vs
-// 3.34e9, DX 4.13e9 vs 3.90e9 histories/s with 5), 7 for palette worlds
-// (DX 4.50e9 vs 4.39e9; CT spiral +0.6 %, whole body -1.3 %, so code worlds
-// keep 6); L = 2 would spill at 6 and keeps 5.
+// 3.34e9, DX 4.13e9 vs 3.90e9 histories/s with 5), 7 for the two fixed
+// world kinds (palette: DX 4.50e9 vs 4.39e9; 1-byte codes with block codes:
+// CT spiral 3.95e9 vs 3.86e9, whole body 1.69e9 vs 1.59e9, their flights
+// being bound by gather latency more than by VALU issue; the 72-VGPR code
+// kernel keeps one lane counter in scratch, touched once per history);
+// L = 2 would spill at 6 and keeps 5.
 uint32_t wavesFp32(uint32_t requested, int L, uint32_t kind)
 {
     if (requested == 0)
-        return L == 2 ? 5u : kind == WORLD_PALETTE8 ? 7u : 6u;
+        return L == 2 ? 5u : kind != WORLD_ANY ? 7u : 6u;
     if (requested >= 7)
         return kind != WORLD_ANY && L < 2 ? 7u : 6u;
     return requested >= 6 ? 6u : 5u;

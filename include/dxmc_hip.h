@@ -218,7 +218,7 @@ typedef struct {
     uint32_t queue_depth; /* FP32: histories a wave claims per refill of its pool, 0: tuned */
     uint32_t waves_per_simd; /* FP32: kernel variant by register budget, 5, 6 or 7 resident waves per SIMD
                               * (7 only at L = 0, 1 on worlds of 1-byte codes or palette indices, else 6);
-                              * 0: 7 on palette worlds and 6 otherwise for L = 0, 1, 5 for L = 2 */
+                              * 0: 7 on those worlds and 6 on others for L = 0, 1, 5 for L = 2 */
     uint32_t lds_tables; /* FP32: sampling tables in LDS, 1 none, 2 all but the attenuation coefficients, 3 all (0: tuned) */
 } dxmc_run_cfg;
 

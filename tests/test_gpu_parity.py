@@ -394,8 +394,9 @@ def test_bounded_steps_exact(device, n_materials, jitter):
     tt = device.upload_tables(t.desc())
     out = []
     try:
-        # bounds, every step in full, and the 7-wave kernel (1-byte codes)
-        for flags, waves in ((capi.RUN_COUNTERS, 0), (capi.RUN_COUNTERS | capi.RUN_EXACT_STEPS, 0), (capi.RUN_COUNTERS, 7)):
+        # bounds (the 7-wave kernel on 1-byte codes), every step in full, and
+        # the 6-wave kernel
+        for flags, waves in ((capi.RUN_COUNTERS, 0), (capi.RUN_COUNTERS | capi.RUN_EXACT_STEPS, 0), (capi.RUN_COUNTERS, 6)):
             cfg = capi.RunCfg()
             cfg.flags = flags
             cfg.waves_per_simd = waves
@@ -448,9 +449,8 @@ def test_block_bounds_exact(device):
     out = []
     try:
         for L in (1, 2):
-            for flags, waves in ((capi.RUN_COUNTERS, 0), (capi.RUN_COUNTERS | capi.RUN_EXACT_STEPS, 0), (capi.RUN_COUNTERS, 7)):
-                if L == 2 and waves == 7:
-                    continue
+            # default waves (7 at L = 1, 5 at L = 2), every step in full, 6 waves
+            for flags, waves in ((capi.RUN_COUNTERS, 0), (capi.RUN_COUNTERS | capi.RUN_EXACT_STEPS, 0), (capi.RUN_COUNTERS, 6)):
                 cfg = capi.RunCfg()
                 cfg.flags = flags
                 cfg.waves_per_simd = waves
@@ -463,6 +463,7 @@ def test_block_bounds_exact(device):
         device.destroy_tables(tt)
         device.destroy_world(w)
     print(f"  steps {[o[1] for o in out]}, events {[o[2] for o in out]}")
+    assert len(out) == 6
     for L in (1, 2):
         runs = [o for o in out if o[0] == L]
         assert runs[0][1] > 0 and runs[0][2] > 0
