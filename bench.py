@@ -48,6 +48,7 @@ def parse():
     p.add_argument("--queue-depth", type=int, default=0, help="FP32 histories per pool refill, 0: default")
     p.add_argument("--waves-per-simd", type=int, default=0, help="FP32 kernel variant (5, 6, 7), 0: default (7 on worlds of 1-byte codes or palette indices and 6 on others for L=0, 1; 5 for L=2)")
     p.add_argument("--lds-tables", type=int, default=0, help="FP32 tables in LDS: 1 none, 2 all but the LUT, 3 all; 0: default")
+    p.add_argument("--block-bounds", type=int, default=0, help="FP32 on 1-byte code worlds: 1 block code ranges first, 2 voxel codes only; 0: default")
     p.add_argument("--low-energy-correction", type=int, default=1)
     p.add_argument("--no-autotune", action="store_true", help="skip the autotuner (defaults or the fields given)")
     p.add_argument("--no-serving", action="store_true", help="skip the request-stream (serving) measurement")
@@ -272,6 +273,7 @@ def main():
     tuning.queue_depth = args.queue_depth
     tuning.waves_per_simd = args.waves_per_simd
     tuning.lds_tables = args.lds_tables
+    tuning.block_bounds = args.block_bounds
     tuning.flags = 1
 
     with backend.Device(0) as dev:
@@ -287,7 +289,8 @@ def main():
             tune_report = {"seconds": time.perf_counter() - t1, "trials": rep.trials, "default_rate": rep.default_rate,
                            "best_rate": rep.best_rate, "regeneration_threshold": tuning.regeneration_threshold,
                            "interaction_threshold": tuning.interaction_threshold, "queue_depth": tuning.queue_depth,
-                           "blocks_per_cu": tuning.blocks_per_cu, "waves_per_simd": tuning.waves_per_simd, "lds_tables": tuning.lds_tables}
+                           "blocks_per_cu": tuning.blocks_per_cu, "waves_per_simd": tuning.waves_per_simd, "lds_tables": tuning.lds_tables,
+                           "block_bounds": tuning.block_bounds}
             print(f"autotune: {tune_report}", file=sys.stderr, flush=True)
         tally = dev.tally(n)
         for i in range(args.warmup):

@@ -145,6 +145,8 @@ class RunCfg(C.Structure):
         ("queue_depth", C.c_uint32),
         ("waves_per_simd", C.c_uint32),
         ("lds_tables", C.c_uint32),
+        ("block_bounds", C.c_uint32),
+        ("reserved0", C.c_uint32),
     ]
 
 

@@ -4,6 +4,7 @@
 // caller's exposures into a scratch tally and picks, one parameter at a
 // time starting from the defaults, the value with the highest histories/s:
 //   FP32 persistent kernel: register budget (resident waves per SIMD),
+//                           LDS staging, block code ranges on or off,
 //                           regeneration threshold, interaction threshold,
 //                           queue depth (histories per pool refill), resident
 //                           blocks per CU
@@ -101,7 +102,7 @@ int autotune(dxmc_ctx* c, const dxmc_world* w, const dxmc_tables* t, const dxmc_
     cfg.flags &= ~DXMC_RUN_AUTOTUNE;
     dxmc_run_cfg defaults = cfg;
     defaults.block_size = defaults.blocks_per_cu = defaults.regeneration_threshold = defaults.interaction_threshold = defaults.queue_depth = 0;
-    defaults.waves_per_simd = defaults.lds_tables = 0;
+    defaults.waves_per_simd = defaults.lds_tables = defaults.block_bounds = 0;
     double defaultRate = tr.rate(defaults, st);
     double bestRate = st ? -1 : tr.rate(cfg, st);
     if (!st) {
@@ -109,6 +110,8 @@ int autotune(dxmc_ctx* c, const dxmc_world* w, const dxmc_tables* t, const dxmc_
             st = searchField(tr, cfg, &dxmc_run_cfg::waves_per_simd, { 5, 6, 7 }, bestRate);
             if (!st)
                 st = searchField(tr, cfg, &dxmc_run_cfg::lds_tables, { 1, 2, 3 }, bestRate);
+            if (!st && w->blocks)
+                st = searchField(tr, cfg, &dxmc_run_cfg::block_bounds, { 1, 2 }, bestRate);
             const uint32_t resident = residentBlocksFp32(cfg.low_energy_correction, wavesFp32(cfg.waves_per_simd, cfg.low_energy_correction, 0), 0);
             std::vector<uint32_t> perCU;
             for (uint32_t b : { resident > 1 ? resident - 1 : 1u, std::max(1u, resident / 2) })

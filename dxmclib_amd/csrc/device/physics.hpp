@@ -160,7 +160,8 @@ struct World {
     // 1 when the linear voxel index fits 24-bit products: nx * ny (the
     // slice stride 'nxy') and nz below 2^24 and the voxel count below 2^32
     uint32_t index24, nxy;
-    // block codes (null without 1-byte codes), blocks per row and per slice
+    // block codes (null without 1-byte codes, or when a run does not use
+    // them), blocks per row and per slice
     const uint16_t* blocks;
     uint32_t bnx, bnxy;
 };

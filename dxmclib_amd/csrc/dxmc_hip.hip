@@ -600,7 +600,8 @@ int dxmc_hip_run_monitored(dxmc_ctx* c, const dxmc_world* w, const dxmc_tables* 
         }
         const dxmc_run_cfg& b = it->second;
         for (auto f : { &dxmc_run_cfg::block_size, &dxmc_run_cfg::blocks_per_cu, &dxmc_run_cfg::regeneration_threshold,
-                 &dxmc_run_cfg::interaction_threshold, &dxmc_run_cfg::queue_depth, &dxmc_run_cfg::waves_per_simd, &dxmc_run_cfg::lds_tables })
+                 &dxmc_run_cfg::interaction_threshold, &dxmc_run_cfg::queue_depth, &dxmc_run_cfg::waves_per_simd, &dxmc_run_cfg::lds_tables,
+                 &dxmc_run_cfg::block_bounds })
             if (run.*f == 0)
                 run.*f = b.*f;
     }
@@ -623,6 +624,8 @@ int validateRun(dxmc_ctx* c, const dxmc_world* w, const dxmc_tables* t, const dx
         return c->fail(DXMC_ERR_ARGUMENT, "run: tally size differs from the world size");
     if (cfg->precision == DXMC_PRECISION_FP64 && cfg->block_size > 256)
         return c->fail(DXMC_ERR_ARGUMENT, "run: FP64 block size must be at most 256");
+    if (cfg->block_bounds > 2)
+        return c->fail(DXMC_ERR_ARGUMENT, "run: block_bounds must be 0, 1 or 2");
     if (t->nMaterials < w->nMaterials)
         return c->fail(DXMC_ERR_ARGUMENT, "run: tables hold fewer materials than the world uses");
     if (cfg->low_energy_correction < 0 || cfg->low_energy_correction > 2 || (cfg->precision != DXMC_PRECISION_FP64 && cfg->precision != DXMC_PRECISION_FP32))

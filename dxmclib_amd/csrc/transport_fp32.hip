@@ -45,9 +45,9 @@ enum : uint32_t { DEAD = 0, FLYING = 1, PENDING = 2 };
 // World kinds the kernel is compiled for: the two common layouts get their
 // code width, record width, 24-bit index and bounds table fixed at compile
 // time, so a flight carries no branches on them
-//   WORLD_CODES8    1-byte bounded-density codes and their block codes,
-//                   8-byte records, the attenuation bounds table, 24-bit
-//                   index (CT-sized worlds)
+//   WORLD_CODES8    1-byte bounded-density codes (and their block codes
+//                   when the run uses them), 8-byte records, the
+//                   attenuation bounds table, 24-bit index (CT-sized worlds)
 //   WORLD_PALETTE8  1-byte palette indices, no codes, 24-bit index
 //   WORLD_ANY       every other world (branches at run time)
 enum : uint32_t { WORLD_ANY = 0, WORLD_CODES8 = 1, WORLD_PALETTE8 = 2 };
@@ -406,7 +406,10 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(W, 8))) voi
                     // the record).
                     float prob = 0.0f;
                     bool settled = false;
-                    if (K == WORLD_CODES8) {
+                    // (the L = 2 kernel also runs without block codes: a run
+                    // switch there costs no occupancy; at L = 0, 1 a world
+                    // without them runs the K = any kernel)
+                    if (K == WORLD_CODES8 && (L < 2 || w.blocks != nullptr)) {
                         const uint32_t bc = w.blocks[blockIndexFast(w, vc)];
                         const float bLo = w.bounds[bc & 0xffu].lo;
                         const uint2 bh = *reinterpret_cast<const uint2*>(&w.bounds[bc >> 8].hi); // hi, material
@@ -522,7 +525,7 @@ uint32_t worldKind(const World<float>& w, const Tables<float>& t)
 {
     if (!w.index24)
         return WORLD_ANY;
-    if (w.codeBytes == 1 && w.voxelBytes == 8 && t.lutSumBound && w.blocks)
+    if (w.codeBytes == 1 && w.voxelBytes == 8 && t.lutSumBound)
         return WORLD_CODES8;
     if (w.codeBytes == 0 && w.voxelBytes == 1 && !t.lutSumBound)
         return WORLD_PALETTE8;
@@ -607,7 +610,22 @@ void launchFp32(dxmc_ctx* c, const dxmc_run_cfg& cfg, const World<float>& w, con
         world.codeBytes = 0;
         tables.lutSumBound = nullptr;
     }
-    const uint32_t kind = worldKind(world, tables);
+    // the block code ranges spare most steps the code gather, and pay
+    // wherever that gather misses the caches or 7 waves per SIMD hide the
+    // block entry's loads (measured: CT spiral +3 % at L = 1, whole body
+    // +39 %); at L = 2, with 5 waves per SIMD, the longer chain of a step
+    // whose draw falls in the block's band costs more than a code gather
+    // from the Infinity Cache saves (CT spiral -6.5 %, its 79 MB codes
+    // cached; whole body +13 %, its 629 MB not)
+    constexpr uint64_t INFINITY_CACHE_BYTES = uint64_t(256) << 20;
+    const uint64_t codeArrayBytes = uint64_t(world.nx) * world.ny * world.nz * world.codeBytes;
+    const bool blocksOn = world.codeBytes == 1 && world.blocks
+        && (cfg.block_bounds == 1 || (cfg.block_bounds == 0 && !(cfg.low_energy_correction == 2 && codeArrayBytes <= INFINITY_CACHE_BYTES)));
+    if (!blocksOn)
+        world.blocks = nullptr;
+    uint32_t kind = worldKind(world, tables);
+    if (kind == WORLD_CODES8 && !world.blocks && cfg.low_energy_correction < 2)
+        kind = WORLD_ANY;
     const uint32_t waves = wavesFp32(cfg.waves_per_simd, cfg.low_energy_correction, kind);
     const uint32_t stage = stageLevelFp32(cfg, tables, waves, kind);
     const uint32_t lds = ldsBytes(tables, stage, cfg.low_energy_correction) + WAVES_PER_BLOCK * queueBytes(waves);

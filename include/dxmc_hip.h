@@ -220,6 +220,10 @@ typedef struct {
                               * (7 only at L = 0, 1 on worlds of 1-byte codes or palette indices, else 6);
                               * 0: 7 on those worlds and 6 on others for L = 0, 1, 5 for L = 2 */
     uint32_t lds_tables; /* FP32: sampling tables in LDS, 1 none, 2 all but the attenuation coefficients, 3 all (0: tuned) */
+    uint32_t block_bounds; /* FP32 on worlds with 1-byte codes: 1 settle Woodcock decisions from the code range of the
+                            * voxel's 8x8x8 block before its own code, 2 from the voxel's code only (same results);
+                            * 0: 1, except at L = 2 when the code array fits the 256 MB Infinity Cache */
+    uint32_t reserved0; /* 0 */
 } dxmc_run_cfg;
 
 /* dxmc_run_cfg.flags */
@@ -292,7 +296,7 @@ int dxmc_hip_run_noalloc(dxmc_ctx* ctx, const dxmc_world* world, const dxmc_tabl
  * CPU threads need no tuning). Times trial launches of trial_histories
  * histories drawn evenly from the exposure table into a scratch tally and
  * searches, one parameter at a time from the defaults, over
- *   FP32: waves_per_simd, lds_tables, regeneration_threshold,
+ *   FP32: waves_per_simd, lds_tables, block_bounds, regeneration_threshold,
  *         interaction_threshold, queue_depth, blocks_per_cu;
  *   FP64: block_size.
  * Tuning fields that are non-zero in 'base' are kept fixed. 'best' receives

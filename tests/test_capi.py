@@ -37,7 +37,7 @@ def test_struct_layouts():
     assert C.sizeof(capi.Exposure) == 18 * 8 + 8 + 16
     assert C.sizeof(capi.Shell) == 11 * 8
     assert C.sizeof(capi.Heel) == 4 * 8 + 16
-    assert C.sizeof(capi.RunCfg) == 64
+    assert C.sizeof(capi.RunCfg) == 72
 
 
 C_NAMES = {"WorldDesc": "dxmc_world_desc", "Shell": "dxmc_shell", "Heel": "dxmc_heel", "TablesDesc": "dxmc_tables_desc",

@@ -424,7 +424,7 @@ def test_block_bounds_exact(device):
     water slab with a bone ellipsoid and a lung region (blocks of one
     material, blocks on the boundaries holding several, ragged sizes that
     clip the last blocks) runs the same histories as evaluating every step
-    in full."""
+    in full, and as reading every step's code (block_bounds = 2)."""
     print("block bounds", flush=True)
     rng = np.random.default_rng(21)
     shape = (45, 50, 53)  # z, y, x: no size a multiple of 8
@@ -449,11 +449,14 @@ def test_block_bounds_exact(device):
     out = []
     try:
         for L in (1, 2):
-            # default waves (7 at L = 1, 5 at L = 2), every step in full, 6 waves
-            for flags, waves in ((capi.RUN_COUNTERS, 0), (capi.RUN_COUNTERS | capi.RUN_EXACT_STEPS, 0), (capi.RUN_COUNTERS, 6)):
+            # default waves (7 at L = 1, 5 at L = 2), every step in full, 6
+            # waves, the voxel's code only
+            for flags, waves, bb in ((capi.RUN_COUNTERS, 0, 1), (capi.RUN_COUNTERS | capi.RUN_EXACT_STEPS, 0, 0), (capi.RUN_COUNTERS, 6, 1),
+                                     (capi.RUN_COUNTERS, 0, 2)):
                 cfg = capi.RunCfg()
                 cfg.flags = flags
                 cfg.waves_per_simd = waves
+                cfg.block_bounds = bb
                 tally = device.tally(n)
                 stats = device.run(w, tt, exps, tally, capi.PRECISION_FP32, L, seed=5, tuning=cfg)
                 d, ev, v = device.download(tally, n)
@@ -463,7 +466,7 @@ def test_block_bounds_exact(device):
         device.destroy_tables(tt)
         device.destroy_world(w)
     print(f"  steps {[o[1] for o in out]}, events {[o[2] for o in out]}")
-    assert len(out) == 6
+    assert len(out) == 8
     for L in (1, 2):
         runs = [o for o in out if o[0] == L]
         assert runs[0][1] > 0 and runs[0][2] > 0
