@@ -136,7 +136,9 @@ struct Exposure {
 // block of 8x8x8 voxels (clipped at the grid's upper faces) the smallest
 // code | the largest code << 8 when all its voxels share one material (the
 // codes of a material are ordered by density), else BLOCK_MIXED, which
-// settles nothing (its code MIXED_CODE is no voxel's). The block's interval
+// settles nothing (its smallest code MIXED_CODE is no voxel's; its largest,
+// 0, names a valid material row). A 1-byte code is material * bins + bin:
+// (code * codeMul) >> CODE_MUL_SHIFT is its material. The block's interval
 // settles most decisions without the voxel's code; the code array is read
 // only for the draws that fall inside the block's acceptance band.
 struct DensityBound {
@@ -164,11 +166,13 @@ struct World {
     // them), blocks per row and per slice
     const uint16_t* blocks;
     uint32_t bnx, bnxy;
+    uint32_t codeMul; // 1-byte codes: material of a code by multiply and shift
 };
 
 constexpr uint32_t BLOCK_SHIFT = 3; // 8 voxels per block edge
 constexpr uint32_t MIXED_CODE = 255; // the 1-byte code kept for mixed blocks
-constexpr uint16_t BLOCK_MIXED = MIXED_CODE | MIXED_CODE << 8;
+constexpr uint16_t BLOCK_MIXED = MIXED_CODE; // smallest code 255, largest 0
+constexpr uint32_t CODE_MUL_SHIFT = 20;
 
 // One voxel's accumulators in 32 aligned bytes: the three atomics of a
 // scoring event land in one cache line (separate arrays cost three random

@@ -411,19 +411,31 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(W, 8))) voi
                     // without them runs the K = any kernel)
                     if (K == WORLD_CODES8 && (L < 2 || w.blocks != nullptr)) {
                         const uint32_t bc = w.blocks[blockIndexFast(w, vc)];
+                        // the material from the largest code by arithmetic, so
+                        // the intervals and the segment bounds load together
+                        const uint32_t bm = __umul24(bc >> 8, w.codeMul) >> CODE_MUL_SHIFT;
                         const float bLo = w.bounds[bc & 0xffu].lo;
-                        const uint2 bh = *reinterpret_cast<const uint2*>(&w.bounds[bc >> 8].hi); // hi, material
-                        const float2 sb = t.lutSumBound[__umul24(bh.y, t.lutNodes) + lp.k];
+                        const float bHi = w.bounds[bc >> 8].hi;
+                        const float2 sb = t.lutSumBound[__umul24(bm, t.lutNodes) + lp.k];
                         const float lo = sb.x * bLo * majInv;
-                        const float hi = sb.y * __uint_as_float(bh.x) * majInv;
+                        const float hi = sb.y * bHi * majInv;
                         settled = (bc != BLOCK_MIXED) & ((u2 < lo) | (u2 >= hi));
-                        matMeas = bh.y;
+                        matMeas = bm;
                         density = bLo;
                         prob = lo;
                     }
                     if (!settled) {
                         float dLo, dHi;
-                        if (codeBytes != 0) {
+                        if (K == WORLD_CODES8) {
+                            // material by arithmetic, interval and segment bounds
+                            // loaded together (code worlds carry no measurement flags)
+                            const uint32_t code = static_cast<const uint8_t*>(w.codes)[idx];
+                            matMeas = __umul24(code, w.codeMul) >> CODE_MUL_SHIFT;
+                            const uint2 b = *reinterpret_cast<const uint2*>(w.bounds + code);
+                            dLo = __uint_as_float(b.x);
+                            dHi = __uint_as_float(b.y);
+                            density = dLo;
+                        } else if (codeBytes != 0) {
                             const uint32_t code = codeBytes == 1 ? static_cast<const uint8_t*>(w.codes)[idx] : static_cast<const uint16_t*>(w.codes)[idx];
                             // the interval as one 16-byte load (read field by field,
                             // the compiler splits it into two loads on the chain)
@@ -632,7 +644,11 @@ void launchFp32(dxmc_ctx* c, const dxmc_run_cfg& cfg, const World<float>& w, con
     // default: as many blocks as stay resident on a CU (registers and LDS)
     const uint32_t perCU = cfg.blocks_per_cu ? cfg.blocks_per_cu : residentBlocksFp32(cfg.low_energy_correction, waves, lds, kind);
     const uint32_t regen = cfg.regeneration_threshold ? std::min<uint32_t>(cfg.regeneration_threshold, 64u) : 4u;
-    const uint32_t interact = cfg.interaction_threshold ? std::min<uint32_t>(cfg.interaction_threshold, 64u) : 20u;
+    // pending lanes before an interaction pass: at L = 2 a pass costs more
+    // (the impulse-approximation sampler), so it waits for more lanes (CT
+    // spiral 2.39e9 at 20, 2.49e9 at 28, 2.52e9 at 32, 2.45e9 at 40; at
+    // L = 1, 32 cost 3 % and 24 0.5 %)
+    const uint32_t interact = cfg.interaction_threshold ? std::min<uint32_t>(cfg.interaction_threshold, 64u) : cfg.low_energy_correction == 2 ? 32u : 20u;
     // histories a wave claims per atomic on the global counter
     const uint32_t pool = cfg.queue_depth ? std::clamp<uint32_t>(cfg.queue_depth, 64u, 1u << 16) : 512u;
     uint64_t grid = static_cast<uint64_t>(c->numCU) * perCU;

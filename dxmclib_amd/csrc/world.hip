@@ -330,12 +330,14 @@ float floatAbove(double v)
 }
 
 // Bounded-density codes of an 8-byte-record world without measurement
-// flags: the materials present share the 255 (1-byte codes, up to 64
-// materials; code 255 is kept for mixed blocks) or 65536 (2-byte codes)
-// codes equally; each material's [min, max] density range is cut into that
-// many equal bins, each widened outwards to FP32 values. Leaves codeBytes 0
-// when a voxel's density does not fall in the bin computed for it. With
-// 1-byte codes the block codes follow.
+// flags. With a material map of up to 64 entries the codes are 1 byte: the
+// map's materials share codes 0-254 equally (code 255 is kept for mixed
+// blocks), code = material * bins + bin, so a kernel finds the material of
+// a code by a multiply and a shift (codeMul). Larger maps get 2-byte codes,
+// the 65536 shared equally by the materials present. Each material's
+// [min, max] density range is cut into its bins, each widened outwards to
+// FP32 values. Leaves codeBytes 0 when a voxel's density does not fall in
+// the bin computed for it. With 1-byte codes the block codes follow.
 int buildBoundCodes(dxmc_ctx* c, dxmc_world* w, ScanResult* dev, const ScanResult& host)
 {
     std::vector<uint32_t> present;
@@ -344,11 +346,11 @@ int buildBoundCodes(dxmc_ctx* c, dxmc_world* w, ScanResult* dev, const ScanResul
             present.push_back(m);
     if (present.empty() || w->hasMeasurement)
         return DXMC_OK;
-    const uint32_t codeBytes = present.size() <= 64 ? 1u : 2u;
+    const uint32_t codeBytes = w->nMaterials <= 64 ? 1u : 2u;
     const uint32_t nCodes = codeBytes == 1 ? 256u : 65536u;
-    const uint32_t bins = (codeBytes == 1 ? nCodes - 1 : nCodes) / static_cast<uint32_t>(present.size());
+    const uint32_t bins = codeBytes == 1 ? (nCodes - 1) / w->nMaterials : nCodes / static_cast<uint32_t>(present.size());
     std::vector<dxmc_dev::DensityBound> table(nCodes, dxmc_dev::DensityBound { 1.0f, 0.0f, 0xffffffffu, 0u });
-    if (codeBytes == 1) // a valid material row; the kernels test BLOCK_MIXED before its interval
+    if (codeBytes == 1) // the smallest code of mixed blocks, which the kernels exclude before using it
         table[dxmc_dev::MIXED_CODE] = dxmc_dev::DensityBound { 0.0f, 0.0f, 0u, 0u };
     std::vector<uint32_t> base(SCAN_BLOCK, 0);
     std::vector<double> lower(SCAN_BLOCK, 0.0), scale(SCAN_BLOCK, 0.0);
@@ -357,7 +359,7 @@ int buildBoundCodes(dxmc_ctx* c, dxmc_world* w, ScanResult* dev, const ScanResul
         // FP32 records hold float(density): their range is the rounded range
         const float a = static_cast<float>(fromOrderKey(~host.minKeyInv[m]));
         const float b = static_cast<float>(fromOrderKey(host.maxKey[m]));
-        base[m] = j * bins;
+        base[m] = (codeBytes == 1 ? m : j) * bins;
         lower[m] = a;
         scale[m] = b > a ? bins / (static_cast<double>(b) - a) : 0.0;
         for (uint32_t k = 0; k < bins; ++k) {
@@ -397,6 +399,10 @@ int buildBoundCodes(dxmc_ctx* c, dxmc_world* w, ScanResult* dev, const ScanResul
     w->nCodes = nCodes;
     if (codeBytes != 1)
         return DXMC_OK;
+    // (code * codeMul) >> CODE_MUL_SHIFT = code / bins for every code below
+    // 256: codeMul exceeds 2^20 / bins by less than 1, which adds less than
+    // 256 / 2^20 to a quotient whose fraction is at most 1 - 1/bins
+    w->codeMul = ((uint32_t(1) << dxmc_dev::CODE_MUL_SHIFT) + bins - 1) / bins;
     constexpr uint32_t E = 1u << dxmc_dev::BLOCK_SHIFT;
     uint32_t bd[3];
     for (int i = 0; i < 3; ++i)
