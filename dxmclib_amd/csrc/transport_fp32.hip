@@ -178,10 +178,10 @@ constexpr float LN2F = 0.693147180559945309f;
 // loops; a lane that takes the photon only adds the segment offset and the
 // majorant)
 constexpr int QUEUE_FIELDS = 12;
-// photons a wave's queue holds: 48 in the 7-wave kernels, whose LDS must
-// fit 7 blocks per CU, else 64 (the whole wave samples at once)
-__host__ __device__ constexpr uint32_t queueCap(uint32_t waves) { return waves >= 7 ? 48u : 64u; }
-__host__ __device__ constexpr uint32_t queueBytes(uint32_t waves) { return QUEUE_FIELDS * queueCap(waves) * 4; }
+// photons a wave's queue holds: 64 (the whole wave samples at once), or in
+// the 7-wave kernels, whose LDS must fit 7 blocks per CU, a capacity the
+// launch passes (48, or less where the staged tables are larger)
+__host__ __device__ constexpr uint32_t queueBytes(uint32_t capacity) { return QUEUE_FIELDS * capacity * 4; }
 constexpr uint32_t WAVES_PER_BLOCK = 4;
 
 // inside the safe extent, without short-circuit branches
@@ -210,7 +210,7 @@ __device__ inline bool roulette(Particle<float>& p, Rng& rng)
 template <int L, int W, uint32_t S, uint32_t K>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(W, 8))) void transportPersistent(World<float> w, Tables<float> tg, const Exposure<float>* exps, const uint64_t* expStart,
     uint32_t nExp, uint64_t hBegin, uint64_t hEnd, uint64_t seed, uint64_t firstHistory, Tally tally, Counters* ctr, unsigned long long* next,
-    uint32_t regenThreshold, uint32_t interactThreshold, uint32_t poolSize)
+    uint32_t regenThreshold, uint32_t interactThreshold, uint32_t poolSize, uint32_t queueCapacity)
 {
     extern __shared__ __align__(16) unsigned char lds[];
     Tables<float> t = tg;
@@ -249,8 +249,9 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(W, 8))) voi
     const uint32_t lane = __lane_id();
 
     // the wave's queue of new photons, already in the world (LDS, SoA)
-    constexpr uint32_t QUEUE_CAP = queueCap(W);
-    float* q = reinterpret_cast<float*>(lds + ldsBytes(tg, S, L) + (threadIdx.x / 64) * queueBytes(W));
+    // (fixed at 48 on palette worlds, where a run-time capacity cost DX 2 %)
+    const uint32_t QUEUE_CAP = W < 7 ? 64u : K == WORLD_PALETTE8 ? 48u : queueCapacity;
+    float* q = reinterpret_cast<float*>(lds + ldsBytes(tg, S, L) + (threadIdx.x / 64) * queueBytes(QUEUE_CAP));
     uint32_t queued = 0;
 
     for (;;) {
@@ -500,7 +501,7 @@ namespace {
 __global__ void clearCounter(unsigned long long* counter) { *counter = 0; }
 
 using KernelFn = void (*)(World<float>, Tables<float>, const Exposure<float>*, const uint64_t*, uint32_t, uint64_t, uint64_t, uint64_t, uint64_t,
-    Tally, Counters*, unsigned long long*, uint32_t, uint32_t, uint32_t);
+    Tally, Counters*, unsigned long long*, uint32_t, uint32_t, uint32_t, uint32_t);
 
 // waves: resolved by wavesFp32 (5, 6 or 7); the 7-wave kernel (72 VGPRs)
 // exists for the two fixed world kinds at L = 0, 1
@@ -576,6 +577,22 @@ uint32_t residentBlocksFp32(int L, uint32_t waves, uint32_t ldsPerBlock, uint32_
     return static_cast<uint32_t>(resident);
 }
 
+// queue capacity of a run's kernel: 64, and in the 7-wave kernels the
+// largest of 48, 40 and 32 photons with which 7 blocks stay resident on a
+// CU beside the staged tables (48 when none does, and always 48 on palette
+// worlds; whole body, 8 materials: 40, 1.87e9 -> 1.94e9 histories/s)
+uint32_t queueCapFp32(int L, uint32_t waves, uint32_t tablesBytes, uint32_t kind)
+{
+    if (waves < 7)
+        return 64;
+    if (kind == WORLD_PALETTE8)
+        return 48;
+    for (uint32_t cap : { 48u, 40u, 32u })
+        if (residentBlocksFp32(L, waves, tablesBytes + WAVES_PER_BLOCK * queueBytes(cap), kind) >= 7)
+            return cap;
+    return 48;
+}
+
 // staging level of a run: the requested one (lowered until it fits), or by
 // default the attenuation coefficients too when they cost no resident block,
 // else the sampling tables when they cost at most one, else none
@@ -583,7 +600,10 @@ uint32_t stageLevelFp32(const dxmc_run_cfg& cfg, const Tables<float>& t, uint32_
 {
     // a block gets at most 64 KiB of dynamic LDS
     constexpr uint32_t ldsPerBlock = 64 * 1024;
-    auto bytes = [&](uint32_t level) { return uint64_t(ldsBytes(t, level, cfg.low_energy_correction)) + WAVES_PER_BLOCK * queueBytes(waves); };
+    auto bytes = [&](uint32_t level) {
+        const uint32_t tb = ldsBytes(t, level, cfg.low_energy_correction);
+        return uint64_t(tb) + WAVES_PER_BLOCK * queueBytes(queueCapFp32(cfg.low_energy_correction, waves, tb, kind));
+    };
     auto fits = [&](uint32_t level) { return bytes(level) <= ldsPerBlock; };
     if (cfg.lds_tables >= STAGE_NONE && cfg.lds_tables <= STAGE_ALL) {
         uint32_t level = cfg.lds_tables;
@@ -640,7 +660,9 @@ void launchFp32(dxmc_ctx* c, const dxmc_run_cfg& cfg, const World<float>& w, con
         kind = WORLD_ANY;
     const uint32_t waves = wavesFp32(cfg.waves_per_simd, cfg.low_energy_correction, kind);
     const uint32_t stage = stageLevelFp32(cfg, tables, waves, kind);
-    const uint32_t lds = ldsBytes(tables, stage, cfg.low_energy_correction) + WAVES_PER_BLOCK * queueBytes(waves);
+    const uint32_t tablesBytes = ldsBytes(tables, stage, cfg.low_energy_correction);
+    const uint32_t cap = queueCapFp32(cfg.low_energy_correction, waves, tablesBytes, kind);
+    const uint32_t lds = tablesBytes + WAVES_PER_BLOCK * queueBytes(cap);
     // default: as many blocks as stay resident on a CU (registers and LDS)
     const uint32_t perCU = cfg.blocks_per_cu ? cfg.blocks_per_cu : residentBlocksFp32(cfg.low_energy_correction, waves, lds, kind);
     const uint32_t regen = cfg.regeneration_threshold ? std::min<uint32_t>(cfg.regeneration_threshold, 64u) : 4u;
@@ -655,6 +677,6 @@ void launchFp32(dxmc_ctx* c, const dxmc_run_cfg& cfg, const World<float>& w, con
     const uint64_t lanesNeeded = (he - hb + 63) / 64 * 64;
     grid = std::max<uint64_t>(1, std::min<uint64_t>(grid, (lanesNeeded + block - 1) / block));
     hipLaunchKernelGGL(kernelFor(cfg.low_energy_correction, waves, stage, kind), dim3(static_cast<uint32_t>(grid)), dim3(block), lds, c->stream, world, tables,
-        exps, expStart, nExp, hb, he, cfg.seed, cfg.first_history, tally, ctr, nx, regen, interact, pool);
+        exps, expStart, nExp, hb, he, cfg.seed, cfg.first_history, tally, ctr, nx, regen, interact, pool, cap);
 }
 }
