@@ -360,22 +360,26 @@ def test_fp32_voxel_records(device, records):
     assert abs(s32.events - s64.events) <= 1e-3 * s64.events
 
 
-@pytest.mark.parametrize("n_materials,jitter", [(3, True), (70, True), (3, False)])
-def test_bounded_steps_exact(device, n_materials, jitter):
+@pytest.mark.parametrize("n_materials,jitter,used", [(3, True, None), (70, True, None), (3, False, None), (64, True, None),
+                                                     (12, True, (0, 5, 11))])
+def test_bounded_steps_exact(device, n_materials, jitter, used):
     """The FP32 kernel settles most Woodcock decisions from bounds: the total
     attenuation bounded over the photon's LUT segment, and the density
     bounded by a 1-byte (up to 64 materials) or 2-byte code per voxel in
     worlds with 8-byte records (material and an interval holding the
     density), or read exactly from the palette. Only draws between the
     bounds evaluate the attenuation and read the record. The transport is
-    identical to evaluating every step in full (DXMC_RUN_EXACT_STEPS)."""
-    print(f"bounded steps, {n_materials} materials, jitter {jitter}", flush=True)
+    identical to evaluating every step in full (DXMC_RUN_EXACT_STEPS). A
+    1-byte code is material * bins + bin over the whole material map (64
+    entries: 3 bins each; a map whose voxels use only some materials keeps
+    bins for the others), and the kernel derives the material from it."""
+    print(f"bounded steps, {n_materials} materials, jitter {jitter}, used {used}", flush=True)
     rng = np.random.default_rng(8)
     shape = (48, 48, 48)
     names = (NAMES * 20)[:n_materials]
     dens_of = np.array((MAXD * 20)[:n_materials], dtype=np.float32)
     dens_of[dens_of < 0.01] = 0.5  # no near-vacuum: every material interacts
-    mat = rng.integers(0, n_materials, shape).astype(np.uint8)
+    mat = rng.integers(0, n_materials, shape).astype(np.uint8) if used is None else np.array(used, np.uint8)[rng.integers(0, len(used), shape)]
     dens = (dens_of[mat] * (rng.uniform(0.9, 1.1, shape) if jitter else 1.0)).astype(np.float32)
     spacing = (1.0, 1.0, 1.0)
     ext = backend.world_extent(shape[::-1], spacing)
