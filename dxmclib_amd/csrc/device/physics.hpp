@@ -133,7 +133,8 @@ struct Exposure {
 // Woodcock decision is settled from the interval at most steps, and the
 // 8-byte record is read only when the draw falls inside the interval's
 // acceptance band. A world with 1-byte codes also carries 'blocks': per
-// block of 8x8x8 voxels (clipped at the grid's upper faces) the smallest
+// block of 2^blockShift voxels per edge (8 to 64, the smallest whose table
+// fits 64 KiB; clipped at the grid's upper faces) the smallest
 // code | the largest code << 8 when all its voxels share one material (the
 // codes of a material are ordered by density), else BLOCK_MIXED, which
 // settles nothing (its smallest code MIXED_CODE is no voxel's; its largest,
@@ -165,11 +166,12 @@ struct World {
     // block codes (null without 1-byte codes, or when a run does not use
     // them), blocks per row and per slice
     const uint16_t* blocks;
-    uint32_t bnx, bnxy;
+    uint32_t bnx, bnxy, blockShift;
     uint32_t codeMul; // 1-byte codes: material of a code by multiply and shift
 };
 
-constexpr uint32_t BLOCK_SHIFT = 3; // 8 voxels per block edge
+constexpr uint32_t BLOCK_SHIFT_MIN = 3, BLOCK_SHIFT_MAX = 6; // 8 to 64 voxels per block edge
+constexpr uint64_t BLOCK_TABLE_BYTES = 64 * 1024; // the largest table the block edge is widened to avoid
 constexpr uint32_t MIXED_CODE = 255; // the 1-byte code kept for mixed blocks
 constexpr uint16_t BLOCK_MIXED = MIXED_CODE; // smallest code 255, largest 0
 constexpr uint32_t CODE_MUL_SHIFT = 20;

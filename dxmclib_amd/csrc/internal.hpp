@@ -224,6 +224,7 @@ struct dxmc_world {
     // | largest code << 8 of a one-material block, else BLOCK_MIXED)
     void* blocks = nullptr;
     uint32_t blockDims[3] = { 0, 0, 0 };
+    uint32_t blockShift = 0; // block edge 2^blockShift voxels
     uint32_t codeMul = 0; // 1-byte codes: (code * codeMul) >> CODE_MUL_SHIFT is the code's material
     uint64_t bytes = 0;
 };
@@ -335,6 +336,7 @@ inline dxmc_dev::World<T> deviceWorld(const dxmc_world* w)
     d.blocks = static_cast<const uint16_t*>(w->blocks);
     d.bnx = w->blockDims[0];
     d.bnxy = w->blockDims[0] * w->blockDims[1];
+    d.blockShift = w->blockShift;
     d.codeMul = w->codeMul;
     d.voxelBytes = w->voxelBytes;
     return d;

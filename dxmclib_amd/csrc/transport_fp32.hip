@@ -78,7 +78,7 @@ __device__ inline uint64_t voxelIndexFast(const World<float>& w, uint3 v, bool i
 // block of a voxel (1-byte code worlds: 24-bit products hold)
 __device__ inline uint32_t blockIndexFast(const World<float>& w, uint3 v)
 {
-    return __umul24(v.z >> BLOCK_SHIFT, w.bnxy) + __umul24(v.y >> BLOCK_SHIFT, w.bnx) + (v.x >> BLOCK_SHIFT);
+    return __umul24(v.z >> w.blockShift, w.bnxy) + __umul24(v.y >> w.blockShift, w.bnx) + (v.x >> w.blockShift);
 }
 
 // ------------------------------------------------------ LDS-staged tables

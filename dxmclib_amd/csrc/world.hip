@@ -18,9 +18,9 @@
 //                   its material and a density interval (Woodcock decisions
 //                   settled from the interval at most steps)
 //   blockCodes      with 1-byte codes, the code range of every block of
-//                   8x8x8 voxels that holds one material (decisions settled
-//                   from the block's interval without reading the voxel's
-//                   code)
+//                   8^3 to 64^3 voxels that holds one material (decisions
+//                   settled from the block's interval without reading the
+//                   voxel's code)
 // The host reads back a few hundred bytes (maxima, flags, distinct count),
 // so the cost of an upload is the copy and a few HBM passes.
 #include "dxmc_hip.h"
@@ -270,38 +270,47 @@ __global__ __launch_bounds__(SCAN_BLOCK) void boundCodes(const uint2* __restrict
     }
 }
 
-// Block code of every block of 8x8x8 voxels of a 1-byte code world (the
-// blocks of the last row, column and slice clipped to the grid): smallest
-// code | largest code << 8 when every voxel of the block has the same
-// material, else BLOCK_MIXED. A material's codes are its density bins in
-// ascending order, so the interval [lo of the smallest, hi of the largest]
-// holds every density of the block.
+// Block code of every block of 2^shift voxels per edge of a 1-byte code
+// world (the blocks of the last row, column and slice clipped to the grid),
+// one workgroup per block: smallest code | largest code << 8 when every
+// voxel of the block has the same material, else BLOCK_MIXED. A material's
+// codes are its density bins in ascending order, so the interval [lo of the
+// smallest, hi of the largest] holds every density of the block.
 __global__ __launch_bounds__(SCAN_BLOCK) void blockCodes(const uint8_t* __restrict__ codes, const dxmc_dev::DensityBound* __restrict__ bounds,
-    uint32_t nx, uint32_t ny, uint32_t nz, uint32_t bnx, uint32_t bny, uint64_t nBlocks, uint16_t* __restrict__ blocks)
+    uint32_t nx, uint32_t ny, uint32_t nz, uint32_t shift, uint32_t bnx, uint32_t bny, uint64_t nBlocks, uint16_t* __restrict__ blocks)
 {
-    constexpr uint32_t E = 1u << dxmc_dev::BLOCK_SHIFT;
-    const uint64_t stride = static_cast<uint64_t>(gridDim.x) * SCAN_BLOCK;
-    for (uint64_t b = static_cast<uint64_t>(blockIdx.x) * SCAN_BLOCK + threadIdx.x; b < nBlocks; b += stride) {
+    __shared__ unsigned int lo, hi, mixed;
+    const uint32_t E = 1u << shift;
+    for (uint64_t b = blockIdx.x; b < nBlocks; b += gridDim.x) {
         const uint32_t bx = static_cast<uint32_t>(b % bnx);
         const uint32_t by = static_cast<uint32_t>((b / bnx) % bny);
         const uint32_t bz = static_cast<uint32_t>(b / (static_cast<uint64_t>(bnx) * bny));
         const uint32_t x0 = bx * E, y0 = by * E, z0 = bz * E;
-        const uint32_t x1 = min(x0 + E, nx), y1 = min(y0 + E, ny), z1 = min(z0 + E, nz);
-        const uint32_t first = codes[(static_cast<uint64_t>(z0) * ny + y0) * nx + x0];
-        const uint32_t material = bounds[first].material;
-        uint32_t lo = first, hi = first;
-        bool mixed = false;
-        for (uint32_t z = z0; z < z1; ++z)
-            for (uint32_t y = y0; y < y1; ++y) {
-                const uint8_t* row = codes + (static_cast<uint64_t>(z) * ny + y) * nx;
-                for (uint32_t x = x0; x < x1; ++x) {
-                    const uint32_t c = row[x];
-                    mixed |= bounds[c].material != material;
-                    lo = min(lo, c);
-                    hi = max(hi, c);
-                }
-            }
-        blocks[b] = mixed ? dxmc_dev::BLOCK_MIXED : static_cast<uint16_t>(lo | hi << 8);
+        const uint32_t w = min(x0 + E, nx) - x0, h = min(y0 + E, ny) - y0, d = min(z0 + E, nz) - z0;
+        const uint32_t material = bounds[codes[(static_cast<uint64_t>(z0) * ny + y0) * nx + x0]].material;
+        if (threadIdx.x == 0) {
+            lo = 255u;
+            hi = 0u;
+            mixed = 0u;
+        }
+        __syncthreads();
+        uint32_t l = 255u, u = 0u, m = 0u;
+        const uint32_t plane = w * h;
+        for (uint32_t i = threadIdx.x; i < plane * d; i += SCAN_BLOCK) {
+            const uint32_t z = z0 + i / plane, r = i % plane;
+            const uint32_t c = codes[(static_cast<uint64_t>(z) * ny + y0 + r / w) * nx + x0 + r % w];
+            m |= bounds[c].material != material ? 1u : 0u;
+            l = min(l, c);
+            u = max(u, c);
+        }
+        atomicMin(&lo, l);
+        atomicMax(&hi, u);
+        if (m)
+            atomicOr(&mixed, 1u);
+        __syncthreads();
+        if (threadIdx.x == 0)
+            blocks[b] = mixed ? dxmc_dev::BLOCK_MIXED : static_cast<uint16_t>(lo | hi << 8);
+        __syncthreads();
     }
 }
 
@@ -403,19 +412,29 @@ int buildBoundCodes(dxmc_ctx* c, dxmc_world* w, ScanResult* dev, const ScanResul
     // 256: codeMul exceeds 2^20 / bins by less than 1, which adds less than
     // 256 / 2^20 to a quotient whose fraction is at most 1 - 1/bins
     w->codeMul = ((uint32_t(1) << dxmc_dev::CODE_MUL_SHIFT) + bins - 1) / bins;
-    constexpr uint32_t E = 1u << dxmc_dev::BLOCK_SHIFT;
-    uint32_t bd[3];
-    for (int i = 0; i < 3; ++i)
-        bd[i] = (w->dims[i] + E - 1) / E;
+    // block edge: the smallest of 8, 16, 32 and 64 voxels whose table fits
+    // 64 KiB, so that it stays in the caches (CT spiral, 512x512x300: 16,
+    // 39 KB, 4.27e9 -> 4.44e9 histories/s against 8; whole body,
+    // 1024x1024x600: 32, 39 KB, 1.95e9 -> 2.09e9; a larger block is more
+    // often mixed, but its entry is far more often cached)
+    uint32_t shift = dxmc_dev::BLOCK_SHIFT_MIN, bd[3];
+    for (;; ++shift) {
+        const uint32_t E = 1u << shift;
+        for (int i = 0; i < 3; ++i)
+            bd[i] = (w->dims[i] + E - 1) / E;
+        if (uint64_t(bd[0]) * bd[1] * bd[2] * sizeof(uint16_t) <= dxmc_dev::BLOCK_TABLE_BYTES || shift == dxmc_dev::BLOCK_SHIFT_MAX)
+            break;
+    }
     const uint64_t nBlocks = uint64_t(bd[0]) * bd[1] * bd[2];
     if ((st = c->alloc(&w->blocks, nBlocks * sizeof(uint16_t))))
         return st;
-    hipLaunchKernelGGL(blockCodes, dim3(gridFor(nBlocks, SCAN_BLOCK, SCAN_GRID)), block, 0, c->stream, static_cast<const uint8_t*>(w->codes), bnd,
-        w->dims[0], w->dims[1], w->dims[2], bd[0], bd[1], nBlocks, static_cast<uint16_t*>(w->blocks));
+    hipLaunchKernelGGL(blockCodes, dim3(gridFor(nBlocks, 1, SCAN_GRID * 4)), block, 0, c->stream, static_cast<const uint8_t*>(w->codes), bnd,
+        w->dims[0], w->dims[1], w->dims[2], shift, bd[0], bd[1], nBlocks, static_cast<uint16_t*>(w->blocks));
     if ((st = c->check(hipGetLastError())))
         return st;
     for (int i = 0; i < 3; ++i)
         w->blockDims[i] = bd[i];
+    w->blockShift = shift;
     return DXMC_OK;
 }
 

@@ -240,10 +240,20 @@ def _spectrum(spec):
     return w, e
 
 
+def _block_edge(shape):
+    """Block edge of a 1-byte code world: the smallest of 8, 16, 32 and 64
+    voxels whose table of 2-byte block codes fits 64 KiB."""
+    for edge in (8, 16, 32, 64):
+        if 2 * int(np.prod([-(-s // edge) for s in shape])) <= 65536:
+            return edge
+    return 64
+
+
 def _block_bytes(shape):
     """Bytes of the 2-byte block codes a 1-byte code world carries: one per
-    block of 8x8x8 voxels, the last row, column and slice clipped."""
-    return 2 * int(np.prod([-(-s // 8) for s in shape]))
+    block, the last row, column and slice clipped."""
+    edge = _block_edge(shape)
+    return 2 * int(np.prod([-(-s // edge) for s in shape]))
 
 
 def _zstats(d1, v1, d2, v2):
@@ -421,22 +431,26 @@ def test_bounded_steps_exact(device, n_materials, jitter, used):
         assert np.allclose(o[2], out[0][2], rtol=1e-10, atol=0)
 
 
-def test_block_bounds_exact(device):
-    """Worlds with 1-byte codes also carry a code range per block of 8x8x8
-    voxels: one-material blocks settle most Woodcock decisions from the
-    block's density interval, without the voxel's code. A world of air, a
-    water slab with a bone ellipsoid and a lung region (blocks of one
-    material, blocks on the boundaries holding several, ragged sizes that
-    clip the last blocks) runs the same histories as evaluating every step
-    in full, and as reading every step's code (block_bounds = 2)."""
-    print("block bounds", flush=True)
+@pytest.mark.parametrize("shape,edge", [((45, 50, 53), 8), ((64, 520, 530), 16)])
+def test_block_bounds_exact(device, shape, edge):
+    """Worlds with 1-byte codes also carry a code range per block of
+    voxels (8 to 64 per edge, the smallest block whose table fits 64 KiB):
+    one-material blocks settle most Woodcock decisions from the block's
+    density interval, without the voxel's code. A world of air, a water
+    slab with a bone ellipsoid and a lung region (blocks of one material,
+    blocks on the boundaries holding several, ragged sizes that clip the
+    last blocks) runs the same histories as evaluating every step in full,
+    and as reading every step's code (block_bounds = 2)."""
+    print(f"block bounds {shape}", flush=True)
+    assert _block_edge(shape) == edge
     rng = np.random.default_rng(21)
-    shape = (45, 50, 53)  # z, y, x: no size a multiple of 8
-    z, y, x = np.meshgrid(*[np.arange(s) for s in shape], indexing="ij")
+    # z, y, x: no size a multiple of the edge; geometry scaled to the grid
+    nz, ny, nx = shape
+    z, y, x = np.ogrid[:nz, :ny, :nx]
     mat = np.zeros(shape, np.uint8)
-    mat[8:40, 5:45, :] = 1
-    mat[((z - 24) / 9.0) ** 2 + ((y - 25) / 12.0) ** 2 + ((x - 20) / 10.0) ** 2 <= 1.0] = 2
-    mat[10:30, 10:26, 36:50] = 3
+    mat[nz * 8 // 45:nz * 40 // 45, ny * 5 // 50:ny * 45 // 50, :] = 1
+    mat[np.broadcast_to(((z - nz * 24 / 45) / (nz * 9 / 45)) ** 2 + ((y - ny / 2) / (ny * 12 / 50)) ** 2 + ((x - nx * 20 / 53) / (nx * 10 / 53)) ** 2 <= 1.0, shape)] = 2
+    mat[nz * 10 // 45:nz * 30 // 45, ny * 10 // 50:ny * 26 // 50, nx * 36 // 53:nx * 50 // 53] = 3
     dens = (np.array([0.001205, 1.0, 1.92, 0.26], np.float32)[mat] * rng.uniform(0.98, 1.02, shape)).astype(np.float32)
     spacing = (1.0, 1.0, 1.0)
     ext = backend.world_extent(shape[::-1], spacing)
@@ -446,7 +460,7 @@ def test_block_bounds_exact(device):
     assert device.world_bytes(w) == n * (1 + 8 + 8 + 1) + 16 * 256 + _block_bytes(shape)
     maxd = device.world_max_density(w, 4)
     t = backend.HostTables(names, maxd, 120.0, 1.0)
-    spec = backend.source_spec(capi.SOURCE_DX, tube_voltage=120.0, al_mm=2.5, sdd=1000.0, field_size=(60.0, 60.0),
+    spec = backend.source_spec(capi.SOURCE_DX, tube_voltage=120.0, al_mm=2.5, sdd=1000.0, field_size=(1.2 * nx, 1.2 * ny),
                                histories_per_exposure=25_000, total_exposures=8)
     exps = t.exposures(spec)
     tt = device.upload_tables(t.desc())
