@@ -171,7 +171,7 @@ struct World {
 };
 
 constexpr uint32_t BLOCK_SHIFT_MIN = 3, BLOCK_SHIFT_MAX = 6; // 8 to 64 voxels per block edge
-constexpr uint64_t BLOCK_TABLE_BYTES = 64 * 1024; // the largest table the block edge is widened to avoid
+constexpr uint64_t BLOCK_TABLE_BYTES = 64 * 1024; // a block-code table above this size takes the next block edge
 constexpr uint32_t MIXED_CODE = 255; // the 1-byte code kept for mixed blocks
 constexpr uint16_t BLOCK_MIXED = MIXED_CODE; // smallest code 255, largest 0
 constexpr uint32_t CODE_MUL_SHIFT = 20;
