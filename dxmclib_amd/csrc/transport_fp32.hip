@@ -75,6 +75,16 @@ __device__ inline uint64_t voxelIndexFast(const World<float>& w, uint3 v, bool i
     return (static_cast<uint64_t>(v.z) * w.ny + v.y) * w.nx + v.x;
 }
 
+// element 'index' of a table addressed by a 32-bit byte offset from its
+// base (tables below 4 GiB): the load takes the base from scalar registers
+// and the offset from one vector register, with no 64-bit address
+// arithmetic per lane
+template <typename T>
+__device__ inline T loadAt(const T* base, uint32_t index)
+{
+    return *reinterpret_cast<const T*>(reinterpret_cast<const unsigned char*>(base) + index * static_cast<uint32_t>(sizeof(T)));
+}
+
 // block of a voxel (1-byte code worlds: 24-bit products hold)
 __device__ inline uint32_t blockIndexFast(const World<float>& w, uint3 v)
 {
@@ -411,13 +421,13 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(W, 8))) voi
                     // switch there costs no occupancy; at L = 0, 1 a world
                     // without them runs the K = any kernel)
                     if (K == WORLD_CODES8 && (L < 2 || w.blocks != nullptr)) {
-                        const uint32_t bc = w.blocks[blockIndexFast(w, vc)];
+                        const uint32_t bc = loadAt(w.blocks, blockIndexFast(w, vc));
                         // the material from the largest code by arithmetic, so
                         // the intervals and the segment bounds load together
                         const uint32_t bm = __umul24(bc >> 8, w.codeMul) >> CODE_MUL_SHIFT;
                         const float bLo = w.bounds[bc & 0xffu].lo;
                         const float bHi = w.bounds[bc >> 8].hi;
-                        const float2 sb = t.lutSumBound[__umul24(bm, t.lutNodes) + lp.k];
+                        const float2 sb = loadAt(t.lutSumBound, __umul24(bm, t.lutNodes) + lp.k);
                         const float lo = sb.x * bLo * majInv;
                         const float hi = sb.y * bHi * majInv;
                         settled = (bc != BLOCK_MIXED) & ((u2 < lo) | (u2 >= hi));
@@ -430,7 +440,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(W, 8))) voi
                         if (K == WORLD_CODES8) {
                             // material by arithmetic, interval and segment bounds
                             // loaded together (code worlds carry no measurement flags)
-                            const uint32_t code = static_cast<const uint8_t*>(w.codes)[idx];
+                            // (1-byte codes: below 2^32 voxels, a 32-bit offset)
+                            const uint32_t code = loadAt(static_cast<const uint8_t*>(w.codes), static_cast<uint32_t>(idx));
                             matMeas = __umul24(code, w.codeMul) >> CODE_MUL_SHIFT;
                             const uint2 b = *reinterpret_cast<const uint2*>(w.bounds + code);
                             dLo = __uint_as_float(b.x);
@@ -451,7 +462,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(W, 8))) voi
                             dHi = density;
                         }
                         if (sumBounds) {
-                            const float2 sb = t.lutSumBound[__umul24(matMeas, t.lutNodes) + lp.k];
+                            const float2 sb = loadAt(t.lutSumBound, __umul24(matMeas, t.lutNodes) + lp.k);
                             const float lo = sb.x * dLo * majInv;
                             const float hi = sb.y * dHi * majInv;
                             settled = (u2 < lo) | (u2 >= hi);
