@@ -480,6 +480,25 @@ def test_block_bounds_exact(device, shape, edge):
                 d, ev, v = device.download(tally, n)
                 device.destroy_tally(tally)
                 out.append((L, stats.steps, stats.events, d, ev))
+        if edge == 8:
+            # the autotuner also tries block_bounds 1 and 2 on such worlds;
+            # its choice runs the same histories
+            best, rep = device.autotune(w, tt, exps, capi.PRECISION_FP32, 1, trial_histories=1 << 18)
+            print(f"  autotune: block_bounds {best.block_bounds}, waves {best.waves_per_simd}", flush=True)
+            best.flags = capi.RUN_COUNTERS
+            tally = device.tally(n)
+            stats = device.run(w, tt, exps, tally, capi.PRECISION_FP32, 1, seed=5, tuning=best)
+            device.destroy_tally(tally)
+            assert (stats.steps, stats.events) == (out[0][1], out[0][2])
+            # block_bounds beyond 2 is refused
+            cfg = capi.RunCfg()
+            cfg.block_bounds = 3
+            tally = device.tally(n)
+            try:
+                with pytest.raises(backend.DxmcError):
+                    device.run(w, tt, exps, tally, capi.PRECISION_FP32, 1, seed=5, tuning=cfg)
+            finally:
+                device.destroy_tally(tally)
     finally:
         device.destroy_tables(tt)
         device.destroy_world(w)
